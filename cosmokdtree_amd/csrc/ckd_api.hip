@@ -1,0 +1,294 @@
+// ckd_api.hip -- the extern "C" entry points declared in
+// include/cosmokdtree_amd.h: argument checks, tree lifetime, error text.
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+
+#include "ckd_internal.h"
+
+namespace ckd {
+int query_knn(const ckd_tree *t, const void *q, int64_t nq, int k, int sorted, const float *seed, void *dist,
+              int64_t *idx, int64_t *stats, hipStream_t s);
+int query_ball(const ckd_tree *t, const void *q, int64_t nq, double r, int64_t *counts, const int64_t *off,
+               void *dist, int64_t *idx, hipStream_t s);
+int query_box(const ckd_tree *t, const void *b, int64_t nb, int64_t *counts, const int64_t *off, int64_t *idx,
+              hipStream_t s);
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+int hip_fail(hipError_t e, const char *what, const char *file, int line) {
+    set_error("HIP error %s (%d) in %s at %s:%d", hipGetErrorString(e), (int)e, what, file, line);
+    return e == hipErrorOutOfMemory ? CKD_ERR_MEMORY : CKD_ERR_HIP;
+}
+
+static void release(ckd_tree *t) {
+    if (t->nodes) (void)hipFree(t->nodes);
+    if (t->bbox) (void)hipFree(t->bbox);
+    if (t->range) (void)hipFree(t->range);
+    if (t->pts) (void)hipFree(t->pts);
+    if (t->idx) (void)hipFree(t->idx);
+    t->nodes = t->bbox = t->pts = t->idx = nullptr;
+    t->range = nullptr;
+}
+
+// CSR offsets from per-row counts held in offsets[1..n]: exclusive scan in
+// place through a temporary buffer, total copied to the host.
+static int counts_to_offsets(int64_t *offsets, int64_t n, int64_t *total, hipStream_t s) {
+    int64_t *tmp = nullptr;
+    void *scr = nullptr;
+    CKD_HIP(hipMalloc(&tmp, sizeof(int64_t) * (n > 0 ? n : 1)));
+    CKD_HIP(hipMalloc(&scr, scan_tmp_bytes(n)));
+    if (n > 0) CKD_HIP(hipMemcpyAsync(tmp, offsets, sizeof(int64_t) * n, hipMemcpyDeviceToDevice, s));
+    int rc = scan_i64(tmp, offsets, n, true, scr, s);
+    int64_t tot = 0;
+    if (!rc) {
+        hipError_t e = hipMemcpyAsync(&tot, offsets + n, sizeof(int64_t), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) rc = hip_fail(e, "offsets total", __FILE__, __LINE__);
+    }
+    (void)hipFree(tmp);
+    (void)hipFree(scr);
+    if (total) *total = tot;
+    return rc;
+}
+
+}  // namespace ckd
+
+using namespace ckd;
+
+extern "C" {
+
+const char *ckd_last_error(void) { return g_err; }
+
+const char *ckd_version(void) { return "cosmokdtree_amd 0.1 (gfx950)"; }
+
+int ckd_build(const void *points, int64_t n, int32_t ndim, int32_t prec, int32_t leaf, const double *boxsize,
+              int32_t idx_bits, void *stream, ckd_tree **out) {
+    g_err[0] = 0;
+    if (!out) {
+        set_error("out must not be NULL");
+        return CKD_ERR_ARG;
+    }
+    *out = nullptr;
+    if (ndim < 1 || ndim > CKD_MAX_DIM) {
+        set_error("Input points must have between 1 and %d dimensions, got %d", CKD_MAX_DIM, ndim);
+        return CKD_ERR_ARG;
+    }
+    if (prec != 4 && prec != 8) {
+        set_error("prec must be 4 or 8, got %d", prec);
+        return CKD_ERR_ARG;
+    }
+    if (leaf < 1) {
+        set_error("Leaf size must be a positive integer");
+        return CKD_ERR_ARG;
+    }
+    if (n < 1 || !points) {
+        set_error("at least one point is required");
+        return CKD_ERR_ARG;
+    }
+    if (idx_bits == 0) idx_bits = n < INT32_MAX ? 32 : 64;
+    if (idx_bits != 32 && idx_bits != 64) {
+        set_error("idx_bits must be 0, 32 or 64");
+        return CKD_ERR_ARG;
+    }
+    if (idx_bits == 32 && n >= INT32_MAX) {
+        set_error("idx_bits=32 cannot index %lld points", (long long)n);
+        return CKD_ERR_LIMIT;
+    }
+    ckd_tree *t = new ckd_tree();
+    t->n = n;
+    t->ndim = ndim;
+    t->prec = prec;
+    t->leafsize = leaf;
+    t->idx_bits = idx_bits;
+    if (boxsize) {
+        t->periodic = 1;
+        for (int d = 0; d < ndim; d++) {
+            if (!(boxsize[d] > 0)) {
+                delete t;
+                set_error("boxsize entries must be positive, got %g on axis %d", boxsize[d], d);
+                return CKD_ERR_ARG;
+            }
+            t->L[d] = boxsize[d];
+        }
+    }
+    (void)hipGetDevice(&t->device);
+    int rc = build_tree(t, points, (hipStream_t)stream);
+    if (rc) {
+        release(t);
+        delete t;
+        return rc;
+    }
+    *out = t;
+    return CKD_OK;
+}
+
+int ckd_free(ckd_tree *t) {
+    if (!t) return CKD_OK;
+    release(t);
+    delete t;
+    return CKD_OK;
+}
+
+int ckd_get_info(const ckd_tree *t, ckd_tree_info *info) {
+    if (!t || !info) {
+        set_error("tree and info must not be NULL");
+        return CKD_ERR_ARG;
+    }
+    memset(info, 0, sizeof(*info));
+    info->n = t->n;
+    info->ndim = t->ndim;
+    info->prec = t->prec;
+    info->leafsize = t->leafsize;
+    info->periodic = t->periodic;
+    info->idx_bits = t->idx_bits;
+    info->depth = t->depth;
+    info->nnodes = t->nnodes;
+    info->nleaves = t->nleaves;
+    for (int d = 0; d < CKD_MAX_DIM; d++) info->boxsize[d] = t->L[d];
+    info->build_ms = t->build_ms;
+    return CKD_OK;
+}
+
+int ckd_knn(const ckd_tree *t, const void *queries, int64_t nq, int32_t k, int32_t sorted, void *dist, int64_t *idx,
+            const float *seed_r, int64_t *stats, void *stream) {
+    g_err[0] = 0;
+    if (!t) {
+        set_error("tree must not be NULL");
+        return CKD_ERR_ARG;
+    }
+    if (k < 1) {
+        set_error("k must be a positive integer, got %d", k);
+        return CKD_ERR_ARG;
+    }
+    if (nq < 0 || (nq > 0 && (!queries || !dist || !idx))) {
+        set_error("invalid query or output arrays");
+        return CKD_ERR_ARG;
+    }
+    if (nq == 0) return CKD_OK;
+    return query_knn(t, queries, nq, k, sorted, seed_r, dist, idx, stats, (hipStream_t)stream);
+}
+
+int ckd_ball_count(const ckd_tree *t, const void *queries, int64_t nq, double radius, int64_t *offsets,
+                   int64_t *total, void *stream) {
+    g_err[0] = 0;
+    if (!t || !offsets) {
+        set_error("tree and offsets must not be NULL");
+        return CKD_ERR_ARG;
+    }
+    if (!(radius > 0)) {
+        set_error("Radius must be a positive number");
+        return CKD_ERR_ARG;
+    }
+    if (nq < 0 || (nq > 0 && !queries)) {
+        set_error("invalid query array");
+        return CKD_ERR_ARG;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    if (nq > 0) {
+        int rc = query_ball(t, queries, nq, radius, offsets + 1, nullptr, nullptr, nullptr, s);
+        if (rc) return rc;
+    }
+    return counts_to_offsets(offsets, nq, total, s);
+}
+
+int ckd_ball_fill(const ckd_tree *t, const void *queries, int64_t nq, double radius, int32_t sorted,
+                  const int64_t *offsets, void *dist, int64_t *idx, void *stream) {
+    g_err[0] = 0;
+    if (!t || !offsets || !(radius > 0)) {
+        set_error("invalid ball_fill arguments");
+        return CKD_ERR_ARG;
+    }
+    if (nq <= 0) return CKD_OK;
+    hipStream_t s = (hipStream_t)stream;
+    int rc = query_ball(t, queries, nq, radius, nullptr, offsets, dist, idx, s);
+    if (rc || !sorted) return rc;
+    return t->prec == 4 ? segmented_sort<float>((float *)dist, idx, offsets, nq, 0, s)
+                        : segmented_sort<double>((double *)dist, idx, offsets, nq, 0, s);
+}
+
+int ckd_box_count(const ckd_tree *t, const void *boxes, int64_t nb, int64_t *offsets, int64_t *total,
+                  void *stream) {
+    g_err[0] = 0;
+    if (!t || !offsets || nb < 0 || (nb > 0 && !boxes)) {
+        set_error("invalid box_count arguments");
+        return CKD_ERR_ARG;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    if (nb > 0) {
+        int rc = query_box(t, boxes, nb, offsets + 1, nullptr, nullptr, s);
+        if (rc) return rc;
+    }
+    return counts_to_offsets(offsets, nb, total, s);
+}
+
+int ckd_box_fill(const ckd_tree *t, const void *boxes, int64_t nb, const int64_t *offsets, int64_t *idx,
+                 void *stream) {
+    g_err[0] = 0;
+    if (!t || !offsets) {
+        set_error("invalid box_fill arguments");
+        return CKD_ERR_ARG;
+    }
+    if (nb <= 0) return CKD_OK;
+    return query_box(t, boxes, nb, nullptr, offsets, idx, (hipStream_t)stream);
+}
+
+int ckd_tree_dump(const ckd_tree *t, int32_t *axis, void *split, void *lo, void *hi, int32_t *child, int32_t *parent,
+                  int64_t *start, int64_t *count, void *points, int64_t *idx) {
+    g_err[0] = 0;
+    if (!t) {
+        set_error("tree must not be NULL");
+        return CKD_ERR_ARG;
+    }
+    const int64_t nn = t->nnodes, n = t->n;
+    const int D = t->ndim, es = t->prec;
+    const size_t nrec = t->prec == 4 ? sizeof(NodeRec<float>) : sizeof(NodeRec<double>);
+    char *hn = (char *)malloc(nrec * nn);
+    char *hb = (char *)malloc((size_t)es * 2 * D * nn);
+    Range *hr = (Range *)malloc(sizeof(Range) * nn);
+    CKD_HIP(hipMemcpy(hn, t->nodes, nrec * nn, hipMemcpyDeviceToHost));
+    CKD_HIP(hipMemcpy(hb, t->bbox, (size_t)es * 2 * D * nn, hipMemcpyDeviceToHost));
+    CKD_HIP(hipMemcpy(hr, t->range, sizeof(Range) * nn, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < nn; i++) {
+        const char *rec = hn + nrec * i;
+        const int32_t *ints = (const int32_t *)(rec + es);
+        if (axis) axis[i] = ints[0];
+        if (child) child[i] = ints[1];
+        if (parent) parent[i] = ints[2];
+        if (split) memcpy((char *)split + (size_t)es * i, rec, es);
+        if (lo) memcpy((char *)lo + (size_t)es * D * i, hb + (size_t)es * 2 * D * i, (size_t)es * D);
+        if (hi) memcpy((char *)hi + (size_t)es * D * i, hb + (size_t)es * (2 * D * i + D), (size_t)es * D);
+        if (start) start[i] = hr[i].start;
+        if (count) count[i] = hr[i].count;
+    }
+    free(hn);
+    free(hb);
+    free(hr);
+    if (points) {
+        const int PS = pstride(D);
+        char *hp = (char *)malloc((size_t)es * PS * n);
+        CKD_HIP(hipMemcpy(hp, t->pts, (size_t)es * PS * n, hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < n; i++) memcpy((char *)points + (size_t)es * D * i, hp + (size_t)es * PS * i, (size_t)es * D);
+        free(hp);
+    }
+    if (idx) {
+        if (t->idx_bits == 64) {
+            CKD_HIP(hipMemcpy(idx, t->idx, sizeof(int64_t) * n, hipMemcpyDeviceToHost));
+        } else {
+            int32_t *h = (int32_t *)malloc(sizeof(int32_t) * n);
+            CKD_HIP(hipMemcpy(h, t->idx, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+            for (int64_t i = 0; i < n; i++) idx[i] = h[i];
+            free(h);
+        }
+    }
+    return CKD_OK;
+}
+
+}  // extern "C"

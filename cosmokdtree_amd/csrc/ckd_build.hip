@@ -1,0 +1,642 @@
+// ckd_build.hip -- level-synchronous sliding-midpoint build on the GPU.
+//
+// Restates build_kdtree / build_kdtree_recursive (cosmokdtree.f90:124-451)
+// one tree level at a time.  Every node of a level that holds more than
+// `leafsize` points ("active") is split in the same pass:
+//   1. its point range is cut into chunks of CH points; one workgroup per
+//      chunk reduces per-axis min/max (with first positions) -- the
+//      minval/maxval/minloc/maxloc of cosmokdtree.f90:349-350,372-373;
+//   2. one workgroup per node combines its chunks and applies the split
+//      rule: longest cell side, exact ties to the larger spread (:355-365),
+//      midpoint 0.5*(lo+hi) (:367), slide to the min (:376-388) or max
+//      (:391-403) when every point lies on one side;
+//   3. a stable prefix-sum partition (points with x < mid first, :458-479;
+//      for a slide only the extreme point moves) scatters coordinates and
+//      indices into the other ping-pong buffer;
+//   4. both children are created as an adjacent pair with their cells
+//      (:418-422); children with <= leafsize points become leaves and their
+//      points are copied to the final AoS leaf buckets.
+// The left part keeps its input order here, while the reference's Lomuto
+// swap permutes the right part; the split sets and therefore the tree are
+// the same whenever the slide's extreme point is unique (the GPU takes the
+// first in its current order).
+#include <vector>
+
+#include "ckd_internal.h"
+
+namespace ckd {
+
+namespace {
+
+constexpr int CH = 2048;          // points per chunk
+constexpr int BT = 256;           // threads per workgroup
+constexpr int IPT = CH / BT;      // points per thread in a chunk
+
+struct Chunk {
+    int64_t begin, end;  // positions in the working buffers
+    int32_t act;         // index into the level's active-node list
+    int32_t pad;
+};
+
+template <typename T>
+struct Ext {  // extreme value + first position
+    T mn, mx;
+    int64_t mnpos, mxpos;
+};
+
+template <typename T>
+__device__ __forceinline__ void ext_merge(Ext<T> &a, const Ext<T> &b) {
+    if (b.mn < a.mn || (b.mn == a.mn && b.mnpos < a.mnpos)) {
+        a.mn = b.mn;
+        a.mnpos = b.mnpos;
+    }
+    if (b.mx > a.mx || (b.mx == a.mx && b.mxpos < a.mxpos)) {
+        a.mx = b.mx;
+        a.mxpos = b.mxpos;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ Ext<T> ext_empty() {
+    Ext<T> e;
+    e.mn = T(INFINITY);
+    e.mx = T(-INFINITY);
+    e.mnpos = INT64_MAX;
+    e.mxpos = INT64_MAX;
+    return e;
+}
+
+template <typename T>
+__device__ __forceinline__ Ext<T> ext_shfl_xor(const Ext<T> &e, int m) {
+    Ext<T> o;
+    o.mn = __shfl_xor(e.mn, m, WAVE);
+    o.mx = __shfl_xor(e.mx, m, WAVE);
+    o.mnpos = __shfl_xor(e.mnpos, m, WAVE);
+    o.mxpos = __shfl_xor(e.mxpos, m, WAVE);
+    return o;
+}
+
+// Reduce one Ext per axis across the block; result valid in thread 0.
+template <typename T, int D>
+__device__ __forceinline__ void block_reduce_ext(Ext<T> (&e)[D], Ext<T> (*lds)[D]) {
+#pragma unroll
+    for (int d = 0; d < D; d++)
+#pragma unroll
+        for (int m = WAVE / 2; m >= 1; m >>= 1) ext_merge(e[d], ext_shfl_xor(e[d], m));
+    const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+    if (lane == 0)
+#pragma unroll
+        for (int d = 0; d < D; d++) lds[w][d] = e[d];
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int i = 1; i < BT / WAVE; i++)
+#pragma unroll
+            for (int d = 0; d < D; d++) ext_merge(e[d], lds[i][d]);
+}
+
+// Decision for one active node.
+template <typename T>
+struct Decision {
+    T mid;
+    int64_t selpos;  // slide: position of the point that moves
+    int32_t axis;
+    int32_t mode;    // 0 partition on x < mid, 1 slide to min, 2 slide to max
+};
+
+template <typename T>
+__device__ __forceinline__ bool goes_left(const Decision<T> &dc, T x, int64_t pos) {
+    return dc.mode == 0 ? (x < dc.mid) : (dc.mode == 1 ? pos == dc.selpos : pos != dc.selpos);
+}
+
+// ---------------------------------------------------------------- kernels
+
+template <typename T, int D>
+__global__ __launch_bounds__(BT) void k_transpose_in(const T *in, int64_t n, T *P, void *I, int idx64) {
+    const int64_t i = (int64_t)blockIdx.x * BT + threadIdx.x;
+    if (i >= n) return;
+#pragma unroll
+    for (int d = 0; d < D; d++) P[(int64_t)d * n + i] = in[i * D + d];
+    if (idx64) ((int64_t *)I)[i] = i;
+    else ((int32_t *)I)[i] = (int32_t)i;
+}
+
+__global__ void k_chunk_count(const int32_t *act, int64_t nact, const Range *range, int64_t *nch) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nact) return;
+    nch[i] = ceil_div(range[act[i]].count, CH);
+}
+
+__global__ void k_chunk_fill(const int32_t *act, const Range *range, const int64_t *chunk_base, Chunk *chunks) {
+    const int64_t a = blockIdx.x;
+    const Range r = range[act[a]];
+    const int64_t nc = ceil_div(r.count, CH), base = chunk_base[a];
+    for (int64_t c = threadIdx.x; c < nc; c += blockDim.x) {
+        Chunk ch;
+        ch.begin = r.start + c * CH;
+        ch.end = r.start + (c + 1) * CH < r.start + r.count ? r.start + (c + 1) * CH : r.start + r.count;
+        ch.act = (int32_t)a;
+        ch.pad = 0;
+        chunks[base + c] = ch;
+    }
+}
+
+template <typename T, int D>
+__global__ __launch_bounds__(BT) void k_chunk_stats(const Chunk *chunks, const int64_t *nchunks, const T *P, int64_t n,
+                                                    Ext<T> *cstat) {
+    __shared__ Ext<T> lds[BT / WAVE][D];
+    if ((int64_t)blockIdx.x >= *nchunks) return;
+    const Chunk ch = chunks[blockIdx.x];
+    Ext<T> e[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) e[d] = ext_empty<T>();
+    for (int64_t p = ch.begin + threadIdx.x; p < ch.end; p += BT) {
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            T x = P[(int64_t)d * n + p];
+            if (x < e[d].mn) { e[d].mn = x; e[d].mnpos = p; }
+            if (x > e[d].mx) { e[d].mx = x; e[d].mxpos = p; }
+        }
+    }
+    block_reduce_ext<T, D>(e, lds);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int d = 0; d < D; d++) cstat[(int64_t)blockIdx.x * D + d] = e[d];
+}
+
+template <typename T, int D>
+__global__ __launch_bounds__(BT) void k_node_decide(const int32_t *act, const int64_t *chunk_base, const int64_t *nch,
+                                                    const Ext<T> *cstat, NodeRec<T> *nodes, T *bbox,
+                                                    Decision<T> *dec, int root_from_data) {
+    __shared__ Ext<T> lds[BT / WAVE][D];
+    const int64_t a = blockIdx.x;
+    const int32_t id = act[a];
+    Ext<T> e[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) e[d] = ext_empty<T>();
+    const int64_t c0 = chunk_base[a], c1 = c0 + nch[a];
+    for (int64_t c = c0 + threadIdx.x; c < c1; c += BT)
+#pragma unroll
+        for (int d = 0; d < D; d++) ext_merge(e[d], cstat[c * D + d]);
+    block_reduce_ext<T, D>(e, lds);
+    if (threadIdx.x != 0) return;
+    T *bb = bbox + (int64_t)id * 2 * D;
+    if (root_from_data) {  // non-periodic root cell = data bounds (cosmokdtree.f90:226-229)
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            bb[d] = e[d].mn;
+            bb[D + d] = e[d].mx;
+        }
+    }
+    T side[D], spread[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        side[d] = bb[D + d] - bb[d];
+        spread[d] = e[d].mx - e[d].mn;
+    }
+    T maxside = side[0];
+    int axis = 0;
+#pragma unroll
+    for (int d = 1; d < D; d++) {
+        if (side[d] > maxside) {
+            maxside = side[d];
+            axis = d;
+        }
+        if (side[d] == maxside && spread[d] > spread[axis]) axis = d;
+    }
+    Decision<T> dc;
+    dc.axis = axis;
+    dc.mid = T(0.5) * (bb[axis] + bb[D + axis]);
+    dc.mode = 0;
+    dc.selpos = -1;
+    if (e[axis].mn >= dc.mid) {
+        dc.mode = 1;
+        dc.mid = e[axis].mn;
+        dc.selpos = e[axis].mnpos;
+    } else if (e[axis].mx < dc.mid) {
+        dc.mode = 2;
+        dc.mid = e[axis].mx;
+        dc.selpos = e[axis].mxpos;
+    }
+    dec[a] = dc;
+    nodes[id].split = dc.mid;
+    nodes[id].axis = axis;
+}
+
+template <typename T>
+__global__ __launch_bounds__(BT) void k_chunk_countleft(const Chunk *chunks, const int64_t *nchunks, int64_t nub,
+                                                        const T *P, int64_t n, const Decision<T> *dec,
+                                                        int64_t *cleft) {
+    __shared__ int64_t lds[BT / WAVE + 1];
+    if ((int64_t)blockIdx.x >= nub) return;
+    if ((int64_t)blockIdx.x >= *nchunks) {
+        if (threadIdx.x == 0) cleft[blockIdx.x] = 0;
+        return;
+    }
+    const Chunk ch = chunks[blockIdx.x];
+    const Decision<T> dc = dec[ch.act];
+    const T *col = P + (int64_t)dc.axis * n;
+    int64_t c = 0;
+    for (int64_t p = ch.begin + threadIdx.x; p < ch.end; p += BT) c += goes_left(dc, col[p], p) ? 1 : 0;
+    int64_t tot;
+    block_exclusive_scan<int64_t, BT>(c, lds, &tot);
+    if (threadIdx.x == 0) cleft[blockIdx.x] = tot;
+}
+
+template <typename T, int D, typename IdxT>
+__global__ __launch_bounds__(BT) void k_chunk_scatter(const Chunk *chunks, const int64_t *nchunks,
+                                                      const int64_t *chunk_base, const int64_t *nch,
+                                                      const int32_t *act, const Range *range, const T *P,
+                                                      const IdxT *I, T *P2, IdxT *I2, int64_t n,
+                                                      const Decision<T> *dec, const int64_t *cleft_off) {
+    __shared__ int64_t lds[BT / WAVE + 1];
+    if ((int64_t)blockIdx.x >= *nchunks) return;
+    const Chunk ch = chunks[blockIdx.x];
+    const Decision<T> dc = dec[ch.act];
+    const int64_t nstart = range[act[ch.act]].start;
+    const int64_t first = chunk_base[ch.act];
+    const int64_t left_before = cleft_off[blockIdx.x] - cleft_off[first];
+    const int64_t n_left = cleft_off[first + nch[ch.act]] - cleft_off[first];
+    const int64_t right_before = (ch.begin - nstart) - left_before;
+    const T *col = P + (int64_t)dc.axis * n;
+    const int64_t p0 = ch.begin + (int64_t)threadIdx.x * IPT;
+    bool f[IPT];
+    int64_t c = 0;
+#pragma unroll
+    for (int j = 0; j < IPT; j++) {
+        const int64_t p = p0 + j;
+        f[j] = p < ch.end && goes_left(dc, col[p], p);
+        c += f[j] ? 1 : 0;
+    }
+    int64_t tot;
+    int64_t lr = block_exclusive_scan<int64_t, BT>(c, lds, &tot);
+#pragma unroll
+    for (int j = 0; j < IPT; j++) {
+        const int64_t p = p0 + j;
+        if (p >= ch.end) break;
+        const int64_t local = p - ch.begin;
+        int64_t dst;
+        if (f[j]) {
+            dst = nstart + left_before + lr;
+            lr++;
+        } else {
+            dst = nstart + n_left + right_before + (local - lr);
+        }
+#pragma unroll
+        for (int d = 0; d < D; d++) P2[(int64_t)d * n + dst] = P[(int64_t)d * n + p];
+        I2[dst] = I[p];
+    }
+}
+
+template <typename T, int D>
+__global__ void k_make_children(const int32_t *act, int64_t nact, const Decision<T> *dec, const int64_t *chunk_base,
+                                const int64_t *nch, const int64_t *cleft_off, NodeRec<T> *nodes, T *bbox,
+                                Range *range, int32_t next_id, int leafsize, int64_t *child_active) {
+    const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= nact) return;
+    const int32_t id = act[a];
+    const Decision<T> dc = dec[a];
+    const int64_t first = chunk_base[a];
+    const int64_t n_left = cleft_off[first + nch[a]] - cleft_off[first];
+    const Range r = range[id];
+    const int32_t c0 = next_id + 2 * (int32_t)a;
+    nodes[id].child = c0;
+    const T *pb = bbox + (int64_t)id * 2 * D;
+    for (int s = 0; s < 2; s++) {
+        const int32_t c = c0 + s;
+        T *cb = bbox + (int64_t)c * 2 * D;
+#pragma unroll
+        for (int d = 0; d < 2 * D; d++) cb[d] = pb[d];
+        if (s == 0) cb[D + dc.axis] = dc.mid;  // left upper bound (cosmokdtree.f90:421)
+        else cb[dc.axis] = dc.mid;             // right lower bound (:422)
+        Range cr;
+        cr.start = s == 0 ? r.start : r.start + n_left;
+        cr.count = s == 0 ? n_left : r.count - n_left;
+        range[c] = cr;
+        NodeRec<T> nr;
+        nr.split = T(0);
+        nr.axis = -1;
+        nr.child = -1;
+        nr.parent = id;
+        nodes[c] = nr;
+        child_active[2 * a + s] = cr.count > leafsize ? 1 : 0;
+    }
+}
+
+__global__ void k_compact_active(const int64_t *child_active, const int64_t *pos, int64_t m, int32_t next_id,
+                                 int32_t *next_act) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    if (child_active[i]) next_act[pos[i]] = next_id + (int32_t)i;
+}
+
+// Copy the points of the leaves created this level (children that are not
+// active) into the final AoS leaf buckets.
+template <typename T, int D, typename IdxT>
+__global__ void k_copy_leaves(const int64_t *child_active, int64_t m, int32_t next_id, const Range *range,
+                              int leafsize, const T *P, const IdxT *I, int64_t n, T *F, IdxT *FI) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t ci = t / leafsize, j = t % leafsize;
+    if (ci >= m || child_active[ci]) return;
+    const Range r = range[next_id + ci];
+    if (j >= r.count) return;
+    const int64_t p = r.start + j;
+    constexpr int PS = pstride(D);
+#pragma unroll
+    for (int d = 0; d < D; d++) F[p * PS + d] = P[(int64_t)d * n + p];
+#pragma unroll
+    for (int d = D; d < PS; d++) F[p * PS + d] = T(0);
+    FI[p] = I[p];
+}
+
+// Root that is already a leaf (n <= leafsize): bounds + copy.
+template <typename T, int D, typename IdxT>
+__global__ __launch_bounds__(BT) void k_root_leaf(const T *P, const IdxT *I, int64_t n, NodeRec<T> *nodes, T *bbox,
+                                                  Range *range, int root_from_data, T *F, IdxT *FI) {
+    __shared__ Ext<T> lds[BT / WAVE][D];
+    Ext<T> e[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) e[d] = ext_empty<T>();
+    for (int64_t p = threadIdx.x; p < n; p += BT)
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            T x = P[(int64_t)d * n + p];
+            if (x < e[d].mn) { e[d].mn = x; e[d].mnpos = p; }
+            if (x > e[d].mx) { e[d].mx = x; e[d].mxpos = p; }
+        }
+    block_reduce_ext<T, D>(e, lds);
+    constexpr int PS = pstride(D);
+    for (int64_t p = threadIdx.x; p < n; p += BT) {
+#pragma unroll
+        for (int d = 0; d < D; d++) F[p * PS + d] = P[(int64_t)d * n + p];
+#pragma unroll
+        for (int d = D; d < PS; d++) F[p * PS + d] = T(0);
+        FI[p] = I[p];
+    }
+    if (threadIdx.x == 0) {
+        if (root_from_data)
+#pragma unroll
+            for (int d = 0; d < D; d++) {
+                bbox[d] = e[d].mn;
+                bbox[D + d] = e[d].mx;
+            }
+        NodeRec<T> nr;
+        nr.split = T(0);
+        nr.axis = -1;
+        nr.child = -1;
+        nr.parent = -1;
+        nodes[0] = nr;
+        Range r;
+        r.start = 0;
+        r.count = n;
+        range[0] = r;
+    }
+}
+
+template <typename T, int D>
+__global__ void k_init_root(NodeRec<T> *nodes, T *bbox, Range *range, int64_t n, TreeView<T> tv, int periodic) {
+    NodeRec<T> nr;
+    nr.split = T(0);
+    nr.axis = -1;
+    nr.child = -1;
+    nr.parent = -1;
+    nodes[0] = nr;
+    Range r;
+    r.start = 0;
+    r.count = n;
+    range[0] = r;
+    if (periodic) {  // cosmokdtree.f90:218-224
+        for (int d = 0; d < D; d++) {
+            bbox[d] = -tv.L[d] / T(2);
+            bbox[D + d] = tv.L[d] / T(2);
+        }
+    }
+}
+
+// Periodic range check (cosmokdtree.f90:176-190): counts points outside.
+template <typename T, int D>
+__global__ void k_check_range(const T *in, int64_t n, TreeView<T> tv, int *bad) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        T x = in[i * D + d];
+        if (x < -tv.L[d] / T(2) || x > tv.L[d] / T(2)) atomicOr(bad, 1);
+    }
+}
+
+// ------------------------------------------------------------ host driver
+
+struct DevBuf {
+    void *p = nullptr;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    int alloc(size_t bytes) {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        hipError_t e = hipMalloc(&p, bytes ? bytes : 16);
+        if (e != hipSuccess) {
+            p = nullptr;
+            set_error("device allocation of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+            return CKD_ERR_MEMORY;
+        }
+        return CKD_OK;
+    }
+    template <typename X>
+    X *as() const { return (X *)p; }
+};
+
+template <typename T, int D, typename IdxT>
+int build_impl(ckd_tree *t, const T *in, hipStream_t s) {
+    const int64_t n = t->n;
+    const int leaf = t->leafsize;
+    const TreeView<T> tvL = view<T>(t);
+    hipEvent_t ev0, ev1;
+    CKD_HIP(hipEventCreate(&ev0));
+    CKD_HIP(hipEventCreate(&ev1));
+    CKD_HIP(hipEventRecord(ev0, s));
+
+    if (t->periodic) {
+        DevBuf bad;
+        int rc = bad.alloc(sizeof(int));
+        if (rc) return rc;
+        CKD_HIP(hipMemsetAsync(bad.p, 0, sizeof(int), s));
+        k_check_range<T, D><<<(unsigned)ceil_div(n, 256), 256, 0, s>>>(in, n, tvL, bad.as<int>());
+        int hbad = 0;
+        CKD_HIP(hipMemcpyAsync(&hbad, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        CKD_HIP(hipStreamSynchronize(s));
+        if (hbad) {
+            set_error("Points outside (-L/2, L/2) range !!");
+            return CKD_ERR_RANGE;
+        }
+    }
+
+    constexpr int PS = pstride(D);
+    DevBuf P, P2, I, I2;
+    int rc;
+    if ((rc = P.alloc(sizeof(T) * D * n)) || (rc = P2.alloc(sizeof(T) * D * n)) ||
+        (rc = I.alloc(sizeof(IdxT) * n)) || (rc = I2.alloc(sizeof(IdxT) * n)))
+        return rc;
+    // final arrays, owned by the tree
+    CKD_HIP(hipMalloc(&t->pts, sizeof(T) * PS * n));
+    CKD_HIP(hipMalloc(&t->idx, sizeof(IdxT) * n));
+
+    int64_t cap = 4 * n / (leaf > 1 ? leaf : 1) + 64;
+    if (cap > 2 * n + 1) cap = 2 * n + 1;
+    auto alloc_nodes = [&](int64_t c) -> int {
+        void *nn = nullptr, *nb = nullptr, *nr = nullptr;
+        CKD_HIP(hipMalloc(&nn, sizeof(NodeRec<T>) * c));
+        CKD_HIP(hipMalloc(&nb, sizeof(T) * 2 * D * c));
+        CKD_HIP(hipMalloc(&nr, sizeof(Range) * c));
+        if (t->nodes) {
+            CKD_HIP(hipMemcpyAsync(nn, t->nodes, sizeof(NodeRec<T>) * t->nnodes, hipMemcpyDeviceToDevice, s));
+            CKD_HIP(hipMemcpyAsync(nb, t->bbox, sizeof(T) * 2 * D * t->nnodes, hipMemcpyDeviceToDevice, s));
+            CKD_HIP(hipMemcpyAsync(nr, t->range, sizeof(Range) * t->nnodes, hipMemcpyDeviceToDevice, s));
+            CKD_HIP(hipStreamSynchronize(s));
+            (void)hipFree(t->nodes);
+            (void)hipFree(t->bbox);
+            (void)hipFree(t->range);
+        }
+        t->nodes = nn;
+        t->bbox = nb;
+        t->range = (Range *)nr;
+        cap = c;
+        return CKD_OK;
+    };
+    t->nnodes = 0;
+    if ((rc = alloc_nodes(cap))) return rc;
+
+    k_transpose_in<T, D><<<(unsigned)ceil_div(n, BT), BT, 0, s>>>(in, n, P.as<T>(), I.p, sizeof(IdxT) == 8);
+    CKD_HIP(hipGetLastError());
+    k_init_root<T, D><<<1, 1, 0, s>>>((NodeRec<T> *)t->nodes, (T *)t->bbox, t->range, n, tvL, t->periodic);
+    t->nnodes = 1;
+    t->depth = 0;
+
+    if (n <= leaf) {
+        k_root_leaf<T, D, IdxT><<<1, BT, 0, s>>>(P.as<T>(), I.as<IdxT>(), n, (NodeRec<T> *)t->nodes, (T *)t->bbox,
+                                                 t->range, !t->periodic, (T *)t->pts, (IdxT *)t->idx);
+        CKD_HIP(hipGetLastError());
+        t->nleaves = 1;
+    } else {
+        // per-level work arrays, sized for the largest possible level
+        const int64_t nact_max = n / (leaf + 1) + 1;
+        const int64_t nchunk_max = n / CH + nact_max + 1;
+        DevBuf act, act2, nch, chunk_base, chunks, cstat, dec, cleft, cleft_off, child_active, child_pos, scratch,
+            dev_nact;
+        if ((rc = act.alloc(sizeof(int32_t) * nact_max)) || (rc = act2.alloc(sizeof(int32_t) * nact_max)) ||
+            (rc = nch.alloc(sizeof(int64_t) * nact_max)) ||
+            (rc = chunk_base.alloc(sizeof(int64_t) * (nact_max + 1))) ||
+            (rc = chunks.alloc(sizeof(Chunk) * nchunk_max)) || (rc = cstat.alloc(sizeof(Ext<T>) * D * nchunk_max)) ||
+            (rc = dec.alloc(sizeof(Decision<T>) * nact_max)) || (rc = cleft.alloc(sizeof(int64_t) * nchunk_max)) ||
+            (rc = cleft_off.alloc(sizeof(int64_t) * (nchunk_max + 1))) ||
+            (rc = child_active.alloc(sizeof(int64_t) * 2 * nact_max)) ||
+            (rc = child_pos.alloc(sizeof(int64_t) * (2 * nact_max + 1))) ||
+            (rc = scratch.alloc(scan_tmp_bytes(2 * nact_max + nchunk_max + 2))))
+            return rc;
+        int32_t zero = 0;
+        CKD_HIP(hipMemcpyAsync(act.p, &zero, sizeof(int32_t), hipMemcpyHostToDevice, s));
+        int64_t nact = 1;
+        int32_t next_id = 1;
+        T *Pc = P.as<T>(), *Pn = P2.as<T>();
+        IdxT *Ic = I.as<IdxT>(), *In = I2.as<IdxT>();
+        int32_t *actc = act.as<int32_t>(), *actn = act2.as<int32_t>();
+        int64_t nleaves = 0;
+        int64_t *h_nact = nullptr;
+        CKD_HIP(hipHostMalloc((void **)&h_nact, sizeof(int64_t), hipHostMallocDefault));
+        int level = 0;
+        while (nact > 0) {
+            if ((int64_t)next_id + 2 * nact > cap) {
+                int64_t nc = cap * 2;
+                while (nc < (int64_t)next_id + 2 * nact) nc *= 2;
+                if (nc > INT32_MAX) {
+                    (void)hipHostFree(h_nact);
+                    set_error("tree needs more than 2^31 nodes");
+                    return CKD_ERR_LIMIT;
+                }
+                t->nnodes = next_id;
+                if ((rc = alloc_nodes(nc))) {
+                    (void)hipHostFree(h_nact);
+                    return rc;
+                }
+            }
+            const unsigned ga = (unsigned)ceil_div(nact, 256);
+            const int64_t nub = n / CH + nact + 1;  // upper bound of chunks this level
+            int64_t *d_nch = nch.as<int64_t>(), *d_cb = chunk_base.as<int64_t>();
+            k_chunk_count<<<ga, 256, 0, s>>>(actc, nact, t->range, d_nch);
+            if ((rc = scan_i64(d_nch, d_cb, nact, true, scratch.p, s))) break;
+            k_chunk_fill<<<(unsigned)nact, 64, 0, s>>>(actc, t->range, d_cb, chunks.as<Chunk>());
+            const int64_t *d_ntot = d_cb + nact;  // total chunks (device)
+            k_chunk_stats<T, D><<<(unsigned)nub, BT, 0, s>>>(chunks.as<Chunk>(), d_ntot, Pc, n, cstat.as<Ext<T>>());
+            k_node_decide<T, D><<<(unsigned)nact, BT, 0, s>>>(actc, d_cb, d_nch, cstat.as<Ext<T>>(),
+                                                              (NodeRec<T> *)t->nodes, (T *)t->bbox,
+                                                              dec.as<Decision<T>>(), level == 0 && !t->periodic);
+            k_chunk_countleft<T><<<(unsigned)nub, BT, 0, s>>>(chunks.as<Chunk>(), d_ntot, nub, Pc, n,
+                                                               dec.as<Decision<T>>(), cleft.as<int64_t>());
+            if ((rc = scan_i64(cleft.as<int64_t>(), cleft_off.as<int64_t>(), nub, true, scratch.p, s))) break;
+            k_chunk_scatter<T, D, IdxT><<<(unsigned)nub, BT, 0, s>>>(
+                chunks.as<Chunk>(), d_ntot, d_cb, d_nch, actc, t->range, Pc, Ic, Pn, In, n, dec.as<Decision<T>>(),
+                cleft_off.as<int64_t>());
+            k_make_children<T, D><<<ga, 256, 0, s>>>(actc, nact, dec.as<Decision<T>>(), d_cb, d_nch,
+                                                     cleft_off.as<int64_t>(), (NodeRec<T> *)t->nodes, (T *)t->bbox,
+                                                     t->range, next_id, leaf, child_active.as<int64_t>());
+            const int64_t m = 2 * nact;
+            if ((rc = scan_i64(child_active.as<int64_t>(), child_pos.as<int64_t>(), m, true, scratch.p, s))) break;
+            k_compact_active<<<(unsigned)ceil_div(m, 256), 256, 0, s>>>(
+                child_active.as<int64_t>(), child_pos.as<int64_t>(), m, next_id, actn);
+            k_copy_leaves<T, D, IdxT><<<(unsigned)ceil_div(m * leaf, 256), 256, 0, s>>>(
+                child_active.as<int64_t>(), m, next_id, t->range, leaf, Pn, In, n, (T *)t->pts, (IdxT *)t->idx);
+            CKD_HIP(hipGetLastError());
+            CKD_HIP(hipMemcpyAsync(h_nact, child_pos.as<int64_t>() + m, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+            CKD_HIP(hipStreamSynchronize(s));
+            const int64_t nact_next = *h_nact;
+            nleaves += m - nact_next;
+            next_id += (int32_t)m;
+            nact = nact_next;
+            std::swap(Pc, Pn);
+            std::swap(Ic, In);
+            std::swap(actc, actn);
+            level++;
+        }
+        (void)hipHostFree(h_nact);
+        if (rc) return rc;
+        t->nnodes = next_id;
+        t->nleaves = nleaves;
+        t->depth = level;
+    }
+    CKD_HIP(hipEventRecord(ev1, s));
+    CKD_HIP(hipEventSynchronize(ev1));
+    float ms = 0;
+    CKD_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+    t->build_ms = ms;
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
+    return CKD_OK;
+}
+
+template <typename T, typename IdxT>
+int build_dispatch(ckd_tree *t, const void *in, hipStream_t s) {
+    switch (t->ndim) {
+        case 1: return build_impl<T, 1, IdxT>(t, (const T *)in, s);
+        case 2: return build_impl<T, 2, IdxT>(t, (const T *)in, s);
+        case 3: return build_impl<T, 3, IdxT>(t, (const T *)in, s);
+        case 4: return build_impl<T, 4, IdxT>(t, (const T *)in, s);
+        case 5: return build_impl<T, 5, IdxT>(t, (const T *)in, s);
+        case 6: return build_impl<T, 6, IdxT>(t, (const T *)in, s);
+        case 7: return build_impl<T, 7, IdxT>(t, (const T *)in, s);
+        case 8: return build_impl<T, 8, IdxT>(t, (const T *)in, s);
+    }
+    set_error("ndim must be in [1, %d], got %d", CKD_MAX_DIM, t->ndim);
+    return CKD_ERR_ARG;
+}
+
+}  // namespace
+
+int build_tree(ckd_tree *t, const void *points, hipStream_t s) {
+    if (t->prec == 4)
+        return t->idx_bits == 64 ? build_dispatch<float, int64_t>(t, points, s)
+                                 : build_dispatch<float, int32_t>(t, points, s);
+    return t->idx_bits == 64 ? build_dispatch<double, int64_t>(t, points, s)
+                             : build_dispatch<double, int32_t>(t, points, s);
+}
+
+}  // namespace ckd

@@ -1,0 +1,224 @@
+// ckd_internal.h -- shared definitions of the gfx950 k-d tree library.
+//
+// Layout of a built tree in HBM (all arrays owned by ckd_tree):
+//   nodes[nnodes]   NodeRec<T> {split, axis, child, parent}: axis < 0 marks a
+//                   leaf; the children of node i are child[i] and child[i]+1
+//                   (allocated as a pair), parent[0] = -1.
+//   bbox[nnodes]    2*D values of T: the node's cell, lo[0..D) then hi[0..D)
+//                   (KDTreeNode%minbounds/maxbounds, cosmokdtree.f90:99-100).
+//   range[nnodes]   {start, count} int64: the node's points are rows
+//                   [start, start+count) of the leaf-ordered arrays below,
+//                   for inner nodes too (a subtree is one contiguous range).
+//   pts[n]          leaf-ordered points, AoS with row stride pstride(D)
+//                   elements (D=3 -> 4, so one 16/32-byte load per point).
+//   idx[n]          original 0-based index of each row, int32 or int64.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cfloat>
+#include <string>
+
+#include "../../include/cosmokdtree_amd.h"
+
+#pragma clang fp contract(off)
+
+namespace ckd {
+
+template <typename T>
+struct alignas(sizeof(T) == 4 ? 16 : 8) NodeRec {
+    T split;
+    int32_t axis;    // split axis, -1 for a leaf
+    int32_t child;   // first child id (the second is child+1), -1 for a leaf
+    int32_t parent;  // -1 for the root
+};
+
+struct Range {
+    int64_t start;
+    int64_t count;
+};
+
+__host__ __device__ constexpr int pstride(int D) { return D == 3 ? 4 : (D <= 4 ? D : 8); }
+
+// Device view of a tree, passed by value to kernels.
+template <typename T>
+struct TreeView {
+    const NodeRec<T> *nodes;
+    const T *bbox;
+    const Range *range;
+    const T *pts;
+    const void *idx;
+    int idx64;
+    T L[CKD_MAX_DIM];  // box size per axis (periodic only)
+};
+
+// ---------------------------------------------------------------- errors
+void set_error(const char *fmt, ...);
+int hip_fail(hipError_t e, const char *what, const char *file, int line);
+#define CKD_HIP(call)                                                  \
+    do {                                                               \
+        hipError_t e_ = (call);                                        \
+        if (e_ != hipSuccess) return hip_fail(e_, #call, __FILE__, __LINE__); \
+    } while (0)
+
+// ---------------------------------------------------- device arithmetic
+// Squared distance exactly as distance() of the reference
+// (cosmokdtree.f90:1441-1465): dx = p - q per axis, minimum image
+// min(|dx|, L - |dx|) when periodic, then ((0 + dx1*dx1) + dx2*dx2) + ...
+// in axis order with one rounding per operation (no FMA contraction).
+template <typename T, int D, bool PER>
+__device__ __forceinline__ T dist2(const T *p, const T *q, const T *L) {
+    T s = T(0);
+#pragma unroll
+    for (int i = 0; i < D; i++) {
+        T dx = p[i] - q[i];
+        if (PER) {
+            T a = dx < T(0) ? -dx : dx;
+            T b = L[i] - a;
+            dx = b < a ? b : a;
+        }
+        s = s + dx * dx;
+    }
+    return s;
+}
+
+// Lower bound of dist2 over every point of the cell [lo, hi].  Per axis the
+// bound is built from the same rounded differences the point distance uses,
+// so it never exceeds the distance computed for any point of the cell
+// (rounding is monotone).  Periodic: the nearer of the direct gap and the
+// wrapped gap L - (far face distance).
+template <typename T, int D, bool PER>
+__device__ __forceinline__ T mindist2(const T *box, const T *q, const T *L) {
+    T s = T(0);
+#pragma unroll
+    for (int i = 0; i < D; i++) {
+        T lo = box[i], hi = box[D + i], d = T(0);
+        if (q[i] < lo) {
+            d = lo - q[i];
+            if (PER) {
+                T w = L[i] - (hi - q[i]);
+                d = w < d ? w : d;
+            }
+        } else if (q[i] > hi) {
+            d = q[i] - hi;
+            if (PER) {
+                T w = L[i] - (q[i] - lo);
+                d = w < d ? w : d;
+            }
+        }
+        d = d > T(0) ? d : T(0);
+        s = s + d * d;
+    }
+    return s;
+}
+
+// Upper bound of dist2 over the cell: per axis max(|lo - q|, |hi - q|) with
+// the same rounded differences, summed like dist2.  The minimum image never
+// exceeds the direct difference, so the bound also holds when periodic.
+template <typename T, int D>
+__device__ __forceinline__ T maxdist2(const T *box, const T *q) {
+    T s = T(0);
+#pragma unroll
+    for (int i = 0; i < D; i++) {
+        T a = box[i] - q[i];
+        T b = box[D + i] - q[i];
+        a = a < T(0) ? -a : a;
+        b = b < T(0) ? -b : b;
+        T m = a > b ? a : b;
+        s = s + m * m;
+    }
+    return s;
+}
+
+template <typename T>
+__device__ __forceinline__ T sqrt_rn(T x);
+template <>
+__device__ __forceinline__ float sqrt_rn<float>(float x) { return __builtin_sqrtf(x); }
+template <>
+__device__ __forceinline__ double sqrt_rn<double>(double x) { return __builtin_sqrt(x); }
+
+__device__ __forceinline__ int64_t load_idx(const void *idx, int idx64, int64_t i) {
+    return idx64 ? ((const int64_t *)idx)[i] : (int64_t)((const int32_t *)idx)[i];
+}
+
+// ----------------------------------------------------- wave / block scans
+constexpr int WAVE = 64;
+
+template <typename V>
+__device__ __forceinline__ V wave_inclusive_scan(V v) {
+    const int lane = threadIdx.x & (WAVE - 1);
+#pragma unroll
+    for (int off = 1; off < WAVE; off <<= 1) {
+        V t = __shfl_up(v, off, WAVE);
+        if (lane >= off) v += t;
+    }
+    return v;
+}
+
+// Exclusive scan over a block of NT threads; returns the block total in
+// *total.  `lds` must hold NT / WAVE values.
+template <typename V, int NT>
+__device__ __forceinline__ V block_exclusive_scan(V v, V *lds, V *total) {
+    const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+    V inc = wave_inclusive_scan(v);
+    if (lane == WAVE - 1) lds[w] = inc;
+    __syncthreads();
+    if (w == 0) {
+        V s = lane < NT / WAVE ? lds[lane] : V(0);
+        V si = wave_inclusive_scan(s);
+        if (lane < NT / WAVE) lds[lane] = si - s;
+        if (lane == NT / WAVE - 1) lds[NT / WAVE] = si;
+    }
+    __syncthreads();
+    V r = inc - v + lds[w];
+    *total = lds[NT / WAVE];
+    __syncthreads();
+    return r;
+}
+
+// -------------------------------------------------------- host helpers
+// Exclusive scan of n int64 values into out[0..n); out[n] receives the
+// total when with_total.  Uses `tmp` scratch of scan_tmp_bytes(n).
+size_t scan_tmp_bytes(int64_t n);
+int scan_i64(const int64_t *in, int64_t *out, int64_t n, bool with_total, void *tmp, hipStream_t s);
+
+// Segmented sort of rows by (key, idx) ascending; row r is
+// [off[r], off[r+1]) of keys/vals.  off == NULL means fixed rows of
+// `width` elements.
+template <typename T>
+int segmented_sort(T *keys, int64_t *vals, const int64_t *off, int64_t nrows, int64_t width, hipStream_t s);
+
+__host__ __device__ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+}  // namespace ckd
+
+// Host-side tree object behind the opaque ckd_tree handle.
+struct ckd_tree {
+    int64_t n = 0;
+    int ndim = 0, prec = 8, leafsize = 16, periodic = 0, idx_bits = 32, depth = 0;
+    int device = 0;
+    int64_t nnodes = 0, nleaves = 0;
+    double L[CKD_MAX_DIM] = {0};
+    double build_ms = 0;
+    void *nodes = nullptr;   // NodeRec<T>[nnodes]
+    void *bbox = nullptr;    // T[nnodes * 2 * ndim]
+    ckd::Range *range = nullptr;
+    void *pts = nullptr;     // T[n * pstride(ndim)]
+    void *idx = nullptr;     // int32 / int64 [n]
+};
+
+namespace ckd {
+template <typename T>
+TreeView<T> view(const ckd_tree *t) {
+    TreeView<T> v;
+    v.nodes = (const NodeRec<T> *)t->nodes;
+    v.bbox = (const T *)t->bbox;
+    v.range = t->range;
+    v.pts = (const T *)t->pts;
+    v.idx = t->idx;
+    v.idx64 = t->idx_bits == 64;
+    for (int i = 0; i < CKD_MAX_DIM; i++) v.L[i] = (T)t->L[i];
+    return v;
+}
+int build_tree(ckd_tree *t, const void *points, hipStream_t s);
+}  // namespace ckd

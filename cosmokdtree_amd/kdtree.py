@@ -1,0 +1,279 @@
+"""Drop-in replacement of cosmokdtree's Python API on MI355X.
+
+Same functions, arguments, defaults, assertions and return conventions as
+src/cosmokdtree/kdtree.py of the reference:
+
+    tree = build_kdtree(points, leaf=None, boxsize=None)
+    deallocate_kdtree(tree)
+    dist, idx = knn_search(tree, target, k, sorted=True)
+    dist_list, idx_list = ball_search(tree, target, radius, sorted=False)
+    idx = box_search(tree, box)
+
+The work runs on the GPU through the C ABI of include/cosmokdtree_amd.h;
+numpy inputs are copied to HBM and results copied back.  Differences from
+the reference, all deliberate:
+
+* Periodicity is chosen at run time: a ``boxsize`` with any entry other
+  than -1 builds a periodic tree (the reference fixes it at compile time
+  with PERIODIC and stops the process when a periodic build gets no box,
+  cosmokdtree2py.f90:48-60).  Each tree keeps its own box size (the
+  reference keeps one module-global L, cosmokdtree.f90:86-88,174).
+* float32 points build a float32 tree (the reference Python path is
+  float64 only, src/Makefile:22-25; float32 reproduces a
+  DOUBLEPRECISION=0 Fortran build).  Anything else is converted to float64.
+* Errors raise ValueError / MemoryError / RuntimeError instead of STOPping
+  the process (cosmokdtree.f90:161,190; cosmokdtree2py.f90:58,276).
+* Sorted results list equal distances in index order (the reference's
+  quicksort leaves their order unspecified); unsorted kNN rows are returned
+  in ascending order rather than heap order.
+
+Device-resident work (no host copies) goes through the KDTree methods
+``knn``, ``ball`` and ``box``, which take and return torch tensors on the
+GPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from ._lib import TreeInfo, check, lib
+
+__all__ = ["build_kdtree", "deallocate_kdtree", "knn_search", "ball_search", "box_search", "KDTree",
+           "box_search_batch"]
+
+_REGISTRY: dict[int, "KDTree"] = {}
+
+
+def _vp(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _stream():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _require_gpu():
+    if not torch.cuda.is_available():
+        raise RuntimeError("cosmokdtree_amd needs a GPU (HIP device); none is visible")
+
+
+class KDTree(int):
+    """Handle of a tree in HBM.
+
+    It is an ``int`` (the native tree address), as the reference's handle is
+    (cosmokdtree2py.f90:70-73), and carries the tree's metadata.
+    """
+
+    def __new__(cls, ptr: int, **meta):
+        obj = super().__new__(cls, ptr)
+        obj.__dict__.update(meta)
+        obj._ptr = C.c_void_p(ptr)
+        obj._alive = True
+        return obj
+
+    # ----------------------------------------------------------- metadata
+    def info(self) -> TreeInfo:
+        self._check_alive()
+        inf = TreeInfo()
+        check(lib.ckd_get_info(self._ptr, C.byref(inf)), "ckd_get_info")
+        return inf
+
+    def _check_alive(self):
+        if not self._alive:
+            raise ValueError("the tree has been deallocated")
+
+    def free(self):
+        if self._alive:
+            check(lib.ckd_free(self._ptr), "ckd_free")
+            self._alive = False
+            _REGISTRY.pop(int(self), None)
+
+    # ------------------------------------------------------------ device API
+    def _queries(self, q) -> torch.Tensor:
+        q = torch.as_tensor(q, dtype=self.torch_dtype, device=self.device)
+        if q.dim() == 1:
+            q = q.reshape(1, -1)
+        if q.dim() != 2 or q.shape[1] != self.ndim:
+            raise ValueError(f"targets must have {self.ndim} columns, got shape {tuple(q.shape)}")
+        return q.contiguous()
+
+    def knn(self, q, k: int, sorted: bool = True, seed_r=None, stats=None):
+        """kNN of device queries: returns (dist (nq,k), idx (nq,k) int64) on the GPU."""
+        self._check_alive()
+        q = self._queries(q)
+        nq = q.shape[0]
+        dist = torch.empty((nq, k), dtype=self.torch_dtype, device=self.device)
+        idx = torch.empty((nq, k), dtype=torch.int64, device=self.device)
+        if nq == 0 or k == 0:
+            return dist, idx
+        if seed_r is not None:
+            seed_r = torch.as_tensor(seed_r, dtype=torch.float32, device=self.device).contiguous()
+            if seed_r.numel() != nq:
+                raise ValueError("seed_r needs one radius per query")
+        check(lib.ckd_knn(self._ptr, _vp(q), nq, int(k), int(bool(sorted)), _vp(dist), _vp(idx), _vp(seed_r),
+                          _vp(stats), _stream()), "knn_search")
+        return dist, idx
+
+    def ball(self, q, radius: float, sorted: bool = False):
+        """Ball search of device queries: CSR (offsets (nq+1), dist, idx) on the GPU."""
+        self._check_alive()
+        q = self._queries(q)
+        nq = q.shape[0]
+        off = torch.empty(nq + 1, dtype=torch.int64, device=self.device)
+        total = C.c_int64(0)
+        check(lib.ckd_ball_count(self._ptr, _vp(q), nq, float(radius), _vp(off), C.byref(total), _stream()),
+              "ball_search")
+        dist = torch.empty(max(total.value, 1), dtype=self.torch_dtype, device=self.device)
+        idx = torch.empty(max(total.value, 1), dtype=torch.int64, device=self.device)
+        check(lib.ckd_ball_fill(self._ptr, _vp(q), nq, float(radius), int(bool(sorted)), _vp(off), _vp(dist),
+                                _vp(idx), _stream()), "ball_search")
+        return off, dist[:total.value], idx[:total.value]
+
+    def box(self, boxes):
+        """Box search of device boxes (nb, 2*ndim): CSR (offsets, idx) on the GPU."""
+        self._check_alive()
+        b = torch.as_tensor(boxes, dtype=self.torch_dtype, device=self.device)
+        if b.dim() == 1:
+            b = b.reshape(1, -1)
+        if b.dim() != 2 or b.shape[1] != 2 * self.ndim:
+            raise ValueError("Error: box size does not match tree dimensionality.")
+        b = b.contiguous()
+        nb = b.shape[0]
+        off = torch.empty(nb + 1, dtype=torch.int64, device=self.device)
+        total = C.c_int64(0)
+        check(lib.ckd_box_count(self._ptr, _vp(b), nb, _vp(off), C.byref(total), _stream()), "box_search")
+        idx = torch.empty(max(total.value, 1), dtype=torch.int64, device=self.device)
+        check(lib.ckd_box_fill(self._ptr, _vp(b), nb, _vp(off), _vp(idx), _stream()), "box_search")
+        return off, idx[:total.value]
+
+    def dump(self) -> dict:
+        """Structure arrays on the host (library node numbering)."""
+        inf = self.info()
+        nn, n, d = inf.nnodes, inf.n, inf.ndim
+        dt = self.np_dtype
+        out = dict(axis=np.empty(nn, np.int32), split=np.empty(nn, dt), lo=np.empty((nn, d), dt),
+                   hi=np.empty((nn, d), dt), child=np.empty(nn, np.int32), parent=np.empty(nn, np.int32),
+                   start=np.empty(nn, np.int64), count=np.empty(nn, np.int64), points=np.empty((n, d), dt),
+                   idx=np.empty(n, np.int64))
+        torch.cuda.synchronize()
+        check(lib.ckd_tree_dump(self._ptr, *[out[k].ctypes.data_as(C.c_void_p) for k in (
+            "axis", "split", "lo", "hi", "child", "parent", "start", "count", "points", "idx")]), "dump")
+        return out
+
+
+def _tree(tree) -> KDTree:
+    if isinstance(tree, KDTree):
+        tree._check_alive()
+        return tree
+    t = _REGISTRY.get(int(tree))
+    if t is None:
+        raise ValueError("not a tree handle returned by build_kdtree")
+    return t
+
+
+def _build(points, leaf: int, boxsize) -> KDTree:
+    """Build from a torch tensor or array-like; boxsize None = non-periodic."""
+    _require_gpu()
+    if isinstance(points, torch.Tensor):
+        dt = torch.float32 if points.dtype == torch.float32 else torch.float64
+        pts = points.to(device="cuda", dtype=dt)
+    else:
+        arr = np.asarray(points)
+        dt = torch.float32 if arr.dtype == np.float32 else torch.float64
+        pts = torch.from_numpy(np.ascontiguousarray(arr, dtype=np.float32 if dt == torch.float32 else np.float64))
+        pts = pts.to("cuda")
+    if pts.dim() != 2:
+        raise ValueError("points must be a 2-D array (npoints, ndim)")
+    pts = pts.contiguous()
+    n, d = pts.shape
+    bs = None
+    if boxsize is not None:
+        bs = (C.c_double * d)(*[float(x) for x in boxsize])
+    out = C.c_void_p()
+    prec = 4 if dt == torch.float32 else 8
+    check(lib.ckd_build(_vp(pts), n, d, prec, int(leaf), bs, 0, _stream(), C.byref(out)), "build_kdtree")
+    tree = KDTree(out.value, n=n, ndim=d, leafsize=int(leaf), torch_dtype=dt,
+                  np_dtype=np.float32 if prec == 4 else np.float64, device=pts.device,
+                  periodic=boxsize is not None,
+                  boxsize=None if boxsize is None else np.asarray(boxsize, dtype=np.float64))
+    _REGISTRY[int(tree)] = tree
+    return tree
+
+
+# --------------------------------------------------------------- drop-in API
+
+def build_kdtree(points, leaf=None, boxsize=None):
+    """src/cosmokdtree/kdtree.py:16-34."""
+    if leaf is None:
+        leaf = 16
+    assert leaf > 0, "Leaf size must be a positive integer"
+    if boxsize is None:
+        boxsize = -1. * np.ones(points.shape[1])
+    assert len(boxsize) == points.shape[1], "Incorrect dimensions for boxsize var. Size must be [Lx, Ly, Lz, ...]"
+    boxsize = np.array(boxsize, dtype=np.float64)
+    periodic = bool(np.any(boxsize != -1.))  # cosmokdtree2py.f90:48-55
+    return _build(points, int(leaf), boxsize if periodic else None)
+
+
+def deallocate_kdtree(tree):
+    """src/cosmokdtree/kdtree.py:37-38."""
+    _tree(tree).free()
+
+
+def _host_targets(t: KDTree, target) -> np.ndarray:
+    target = np.array(target, dtype=t.np_dtype)
+    if len(target.shape) == 1:
+        target = target.reshape(1, -1)
+    if target.shape[1] != t.ndim:
+        raise ValueError(f"targets must have {t.ndim} columns, got {target.shape[1]}")
+    return target
+
+
+def knn_search(tree, target, k, sorted=True):
+    """src/cosmokdtree/kdtree.py:41-53: (dist (ntar,k), idx (ntar,k)), 0-based,
+    empty slots -2."""
+    assert type(k) in [int, np.int32, np.int64], "k must be an integer"
+    t = _tree(tree)
+    target = _host_targets(t, target)
+    if k < 0:
+        raise ValueError("k must not be negative")
+    dist, idx = t.knn(torch.from_numpy(target), int(k), sorted=bool(sorted))
+    dist = dist.cpu().numpy()
+    idx = idx.cpu().numpy()
+    return np.asfortranarray(dist), np.asfortranarray(idx)
+
+
+def ball_search(tree, target, radius, sorted=False):
+    """src/cosmokdtree/kdtree.py:56-79: per-target lists of distances and indices."""
+    assert radius > 0., "Radius must be a positive number"
+    radius = float(radius)
+    t = _tree(tree)
+    target = _host_targets(t, target)
+    off, dist, idx = t.ball(torch.from_numpy(target), radius, sorted=bool(sorted))
+    off = off.cpu().numpy()
+    dist = dist.cpu().numpy()
+    idx = idx.cpu().numpy()
+    ntar = target.shape[0]
+    return ([dist[off[i]:off[i + 1]] for i in range(ntar)], [idx[off[i]:off[i + 1]] for i in range(ntar)])
+
+
+def box_search(tree, box):
+    """src/cosmokdtree/kdtree.py:82-87: indices (0-based) of the points in one
+    box (xmin, xmax, ymin, ymax, ...)."""
+    box = np.array(box, dtype=np.float64)
+    t = _tree(tree)
+    if box.size != 2 * t.ndim:  # cosmokdtree2py.f90:275-277
+        raise ValueError("Error: box size does not match tree dimensionality.")
+    off, idx = t.box(torch.from_numpy(box.reshape(1, -1).astype(t.np_dtype)))
+    return idx.cpu().numpy()
+
+
+def box_search_batch(tree, boxes):
+    """Many boxes in one call (no reference counterpart: the reference takes
+    one box per call).  Returns CSR (offsets (nb+1), idx) as numpy arrays."""
+    t = _tree(tree)
+    boxes = np.asarray(boxes, dtype=t.np_dtype).reshape(-1, 2 * t.ndim)
+    off, idx = t.box(torch.from_numpy(boxes))
+    return off.cpu().numpy(), idx.cpu().numpy()

@@ -1,0 +1,156 @@
+/*
+ * cosmokdtree_amd.h -- C ABI of the MI355X (gfx950) k-d tree library.
+ *
+ * The library replaces the f2py extension module `pycosmokdtree` of the
+ * reference (src/cosmokdtree2py.f90) underneath the Python API of
+ * src/cosmokdtree/kdtree.py.  Every entry point below names the reference
+ * routine it replaces.
+ *
+ * Conventions
+ *  - All array arguments marked [dev] are device pointers (HBM) on the
+ *    current HIP device; [host] arguments are host pointers.  `stream` is a
+ *    hipStream_t (NULL = the default stream); work is enqueued on it and
+ *    the call returns when the work is enqueued, except where stated.
+ *  - Coordinates are row-major (n, ndim) in the tree precision: `prec` = 4
+ *    (float) or 8 (double).  The reference Python path is always 8
+ *    (src/Makefile:22-25); 4 reproduces a DOUBLEPRECISION=0 build.
+ *  - Indices returned to the caller are int64 and 0-based, i.e. what
+ *    src/cosmokdtree/kdtree.py returns after its `idx -= 1`.  An empty kNN
+ *    slot (k > n) has index CKD_EMPTY_SLOT (-2, the value kdtree.py:52 turns
+ *    the Fortran sentinel -1 into) and distance sqrt(FLT_MAX) in `prec`
+ *    (cosmokdtree.f90:517-518,538).
+ *  - Periodic trees take `boxsize` (ndim entries, each > 0); every point
+ *    must lie in [-L/2, L/2] per axis (cosmokdtree.f90:176-190).  Distances
+ *    then use the minimum image per axis (cosmokdtree.f90:1453-1457).
+ *  - Every function returns CKD_OK (0) or a negative error code; the text
+ *    of the last error on the calling thread is given by ckd_last_error().
+ *    The reference STOPs the process instead (cosmokdtree.f90:161,170,190;
+ *    cosmokdtree2py.f90:58,276).
+ */
+#ifndef COSMOKDTREE_AMD_H
+#define COSMOKDTREE_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CKD_OK 0
+#define CKD_ERR_ARG -1     /* invalid argument (ndim, leaf, k, radius, sizes) */
+#define CKD_ERR_RANGE -2   /* periodic point outside [-L/2, L/2] */
+#define CKD_ERR_HIP -3     /* HIP runtime error */
+#define CKD_ERR_MEMORY -4  /* device allocation failed */
+#define CKD_ERR_LIMIT -5   /* a documented size limit was exceeded */
+
+#define CKD_EMPTY_SLOT (-2)
+#define CKD_MAX_DIM 8
+
+typedef struct ckd_tree ckd_tree;
+
+typedef struct {
+    int64_t n;          /* number of points */
+    int32_t ndim;       /* dimensions */
+    int32_t prec;       /* 4 or 8 */
+    int32_t leafsize;   /* maximum points per leaf */
+    int32_t periodic;   /* 1 if built with a boxsize */
+    int32_t idx_bits;   /* stored original-index width, 32 or 64 */
+    int32_t depth;      /* levels below the root (a single leaf has 0) */
+    int64_t nnodes;     /* nodes in the flat node array */
+    int64_t nleaves;
+    double boxsize[CKD_MAX_DIM];
+    double build_ms;    /* device time of the last build, milliseconds */
+} ckd_tree_info;
+
+/*
+ * Build a tree over `points` [dev, (n, ndim) row-major, prec].
+ * Replaces pybuild_kdtree (cosmokdtree2py.f90:29-74) -> build_kdtree
+ * (cosmokdtree.f90:124-238): sliding-midpoint rule, leaves of at most
+ * `leaf` points (default 16 in kdtree.py:19-20), out of place (the input
+ * is not modified).  boxsize [host] = NULL builds a non-periodic tree.
+ * idx_bits = 32 or 64 selects the stored index width (0 = 32 when n < 2^31).
+ * The call synchronises `stream` (the level loop is host driven).
+ */
+int ckd_build(const void *points, int64_t n, int32_t ndim, int32_t prec, int32_t leaf, const double *boxsize,
+              int32_t idx_bits, void *stream, ckd_tree **out);
+
+/* Release a tree.  Replaces pydeallocate_tree (cosmokdtree2py.f90:80-97). */
+int ckd_free(ckd_tree *tree);
+
+int ckd_get_info(const ckd_tree *tree, ckd_tree_info *info);
+
+/*
+ * k nearest neighbours of nq query rows `queries` [dev, (nq, ndim), prec].
+ * Replaces pyknn_search (cosmokdtree2py.f90:103-143) -> knn_search
+ * (cosmokdtree.f90:504-540).  Outputs [dev]: dist (nq, k) in prec, idx
+ * (nq, k) int64.  sorted != 0 orders each row by (distance, index); with
+ * sorted == 0 the row order is unspecified (the reference returns its heap
+ * layout).  The distance of each neighbour is computed bit-for-bit as the
+ * reference's distance() (cosmokdtree.f90:1441-1465).
+ * seed_r [dev, nq floats] or NULL: a per-query search radius proposal; a
+ * query whose seeded search finds fewer than k points is searched again
+ * without it, so results never depend on the seed.
+ * stats [dev, 4 int64] or NULL: accumulates {nodes visited, leaf points
+ * tested, queries re-run after a short seed, queries}.
+ */
+int ckd_knn(const ckd_tree *tree, const void *queries, int64_t nq, int32_t k, int32_t sorted, void *dist,
+            int64_t *idx, const float *seed_r, int64_t *stats, void *stream);
+
+/*
+ * Ball search, first pass: offsets [dev, nq+1 int64] receives the CSR row
+ * offsets of the points with squared distance <= radius^2 (radius is cast
+ * to prec first, cosmokdtree.f90:1019,1126).  *total [host] receives
+ * offsets[nq] (the call synchronises `stream` to read it).
+ * Replaces pyball_search_call_1 (cosmokdtree2py.f90:153-201).
+ */
+int ckd_ball_count(const ckd_tree *tree, const void *queries, int64_t nq, double radius, int64_t *offsets,
+                   int64_t *total, void *stream);
+
+/*
+ * Ball search, second pass: fills dist [dev, total, prec] and idx
+ * [dev, total int64] at the offsets of ckd_ball_count.  sorted != 0 orders
+ * each row by (distance, index).  Replaces pyball_search_call_2
+ * (cosmokdtree2py.f90:203-246) with CSR output instead of a zero-padded
+ * (ntar, nballmax) matrix.
+ */
+int ckd_ball_fill(const ckd_tree *tree, const void *queries, int64_t nq, double radius, int32_t sorted,
+                  const int64_t *offsets, void *dist, int64_t *idx, void *stream);
+
+/*
+ * Box search over nb boxes [dev, (nb, 2*ndim) = (xmin, xmax, ymin, ...)],
+ * first pass (CSR offsets + host total).  Replaces pybox_search_call_1
+ * (cosmokdtree2py.f90:252-287) -> box_search (cosmokdtree.f90:1220-1253),
+ * with the reference's inclusive point test and strict child-visit tests
+ * (cosmokdtree.f90:1296-1302,1309-1325).  No periodic wrap, as in the
+ * reference (cosmokdtree.f90:1260).
+ */
+int ckd_box_count(const ckd_tree *tree, const void *boxes, int64_t nb, int64_t *offsets, int64_t *total,
+                  void *stream);
+
+/* Box search second pass.  Replaces pybox_search_call_2
+   (cosmokdtree2py.f90:289-315). */
+int ckd_box_fill(const ckd_tree *tree, const void *boxes, int64_t nb, const int64_t *offsets, int64_t *idx,
+                 void *stream);
+
+/*
+ * Copy the tree structure to host arrays (any pointer may be NULL), in the
+ * library's node numbering (root = 0, children of node i at child[i] and
+ * child[i] + 1).  Sizes: nnodes for node arrays, nnodes*ndim for lo/hi,
+ * n for points (n*ndim) and idx.  Leaf points of node i occupy rows
+ * [start[i], start[i] + count[i]) of points/idx.  No reference counterpart
+ * beyond the node fields of KDTreeNode (cosmokdtree.f90:93-105); used by
+ * tests and by persistence.
+ */
+int ckd_tree_dump(const ckd_tree *tree, int32_t *axis, void *split, void *lo, void *hi, int32_t *child,
+                  int32_t *parent, int64_t *start, int64_t *count, void *points, int64_t *idx);
+
+/* Text of the last error on this thread ("" if none). */
+const char *ckd_last_error(void);
+
+/* Library version string. */
+const char *ckd_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* COSMOKDTREE_AMD_H */
