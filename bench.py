@@ -1,0 +1,242 @@
+#!/usr/bin/env python3
+"""Benchmark of the k-d tree build and query path on one MI355X.
+
+Workload (the clustered periodic box): N = 2^24 points in a periodic cube
+of side L = 100 (coordinates in [-50, 50]), half uniform, half in halos
+with dN/dn ~ n^-1.9 on [10, 1e5] and sigma = 0.02 n^(1/3), float32,
+leaf size 16.  One step = a self-query kNN with k = 32 (sorted) of all N
+points against the tree, inputs and outputs resident in HBM.  The tree is
+built before the timed steps; its build rate is reported beside.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+
+Prints one JSON line (metric, value = queries/s, roofline of the kNN kernel,
+host-core CPU baseline measured in the same run).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "kNN k=32 queries/s on 16.7M periodic pts, learned-seed vs plain; build Mpts/s"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def knn_bytes_per_query(D, s_c, s_i, k, n, nnodes, q):
+    """Algorithmic HBM bytes per kNN query: query read, result write
+    (distance in tree precision + int64 index) and the tree read once,
+    amortised over the queries."""
+    return D * s_c + k * (s_c + 8) + (n * (D * s_c + s_i) + nnodes * 16) / q
+
+
+def build_bytes_per_point(D, s_c, s_i, n, leaf):
+    """Algorithmic HBM bytes per point of the level-synchronous build: input
+    read, read+write of coordinates and index per level, leaf-bucket write."""
+    levels = int(np.ceil(np.log2(max(n / leaf, 2))))
+    return D * s_c + 2 * (D * s_c + s_i) * levels + (D * s_c + s_i)
+
+
+def cpu_baseline(n, q, k, cores):
+    env = dict(os.environ)
+    env.update(OMP_NUM_THREADS=str(cores), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    cmd = [sys.executable, "-m", "oracle.cpu_baseline", "--n", str(n), "--q", str(q), "--k", str(k)]
+    p = subprocess.run(cmd, cwd=ROOT, env=env, capture_output=True, text=True, timeout=600)
+    if p.returncode != 0:
+        log("cpu baseline failed: " + p.stderr[-2000:])
+        return None
+    return json.loads(p.stdout.strip().splitlines()[-1])
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=1 << 24)
+    ap.add_argument("--k", type=int, default=32)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the host-core baseline")
+    ap.add_argument("--no-extra", action="store_true", help="skip the float64 and ball-search extras")
+    ap.add_argument("--cpu-n", type=int, default=1 << 22)
+    ap.add_argument("--cpu-q", type=int, default=1 << 18)
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="also run one device copy of this size (counter calibration under rocprofv3)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from cosmokdtree_amd import kdtree
+    from cosmokdtree_amd.synthetic import SEED_POINTS, SEED_SUBSET, clustered_periodic
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(1)
+    dev = torch.device("cuda")
+    N, K = args.n, args.k
+    L = 100.0
+    t0 = time.time()
+    pts = clustered_periodic(N, L, 0.5, seed=SEED_POINTS, dtype=np.float32)
+    log(f"generated {N} clustered periodic points in {time.time() - t0:.1f}s")
+    P = torch.from_numpy(pts).to(dev)
+    box = np.full(3, L)
+
+    # ---- build (timed beside the steps)
+    builds = []
+    for i in range(args.warmup + 2):
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        tree = kdtree._build(P, 16, box)
+        e1.record()
+        torch.cuda.synchronize()
+        if i >= args.warmup:
+            builds.append(e0.elapsed_time(e1))
+        if i < args.warmup + 1:
+            tree.free()
+    build_ms = float(np.median(builds))
+    info = tree.info()
+    log(f"build {build_ms:.2f} ms ({N / build_ms / 1e3:.1f} Mpts/s), nodes={info.nnodes} depth={info.depth}")
+
+    if args.calib_copy_mib:
+        a = torch.empty(args.calib_copy_mib << 20, dtype=torch.uint8, device=dev)
+        b = torch.empty_like(a)
+        a.fill_(1)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+
+    # ---- kNN steps
+    for _ in range(args.warmup):
+        d, i = tree.knn(P, K, sorted=True)
+    del d, i
+    torch.cuda.synchronize()
+    stats = torch.zeros(4, dtype=torch.int64, device=dev)
+    d, i = tree.knn(P[:1 << 20], K, sorted=True, stats=stats)  # visit counts on a 1M sample (untimed)
+    st = stats.cpu().numpy()
+    del d, i
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        d, i = tree.knn(P, K, sorted=True)
+    e1.record()
+    torch.cuda.synchronize()
+    total_ms = e0.elapsed_time(e1)
+    ms_step = total_ms / args.steps
+    qps = N / (ms_step / 1e3)
+    log(f"kNN k={K}: {ms_step:.2f} ms/step, {qps / 1e6:.1f} Mq/s")
+    s_c, s_i = 4, (8 if info.idx_bits == 64 else 4)
+    bq = knn_bytes_per_query(3, s_c, s_i, K, N, info.nnodes, N)
+    achieved = bq * N / (ms_step / 1e3) / 1e9
+    del d, i
+
+    extra = dict(build_ms=build_ms, build_mpts_s=N / build_ms / 1e3, tree_nodes=int(info.nnodes),
+                 tree_depth=int(info.depth), nodes_visited_per_query=float(st[0] / max(st[3], 1)),
+                 points_tested_per_query=float(st[1] / max(st[3], 1)),
+                 build_bytes_per_point=build_bytes_per_point(3, 4, s_i, N, 16))
+    extra["build_roofline_frac"] = extra["build_bytes_per_point"] * N / (build_ms / 1e3) / 1e9 / HBM_PEAK_GBS
+
+    if not args.no_extra:
+        # ball search r = 0.2 on a random 1M subset (float32 tree)
+        rng = np.random.Generator(np.random.PCG64(SEED_SUBSET))
+        sub = torch.from_numpy(pts[rng.choice(N, 1 << 20, replace=False)]).to(dev)
+        tree.ball(sub, 0.2)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        off, bd, bi = tree.ball(sub, 0.2)
+        torch.cuda.synchronize()
+        tb = time.perf_counter() - t0
+        extra["ball_r0.2_1M_qps"] = (1 << 20) / tb
+        extra["ball_r0.2_mean_count"] = float(off[-1].item()) / (1 << 20)
+        log(f"ball r=0.2 on 1M: {tb * 1e3:.1f} ms, mean count {extra['ball_r0.2_mean_count']:.1f}")
+        del off, bd, bi, sub
+        # float64 build and kNN
+        tree.free()
+        P64 = P.double()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        t64 = kdtree._build(P64, 16, box)
+        torch.cuda.synchronize()
+        extra["f64_build_mpts_s"] = N / (time.perf_counter() - t0) / 1e6
+        d, i = t64.knn(P64, K, sorted=True)
+        del d, i
+        torch.cuda.synchronize()
+        e0.record()
+        d, i = t64.knn(P64, K, sorted=True)
+        e1.record()
+        torch.cuda.synchronize()
+        extra["f64_knn_qps"] = N / (e0.elapsed_time(e1) / 1e3)
+        log(f"f64: build {extra['f64_build_mpts_s']:.1f} Mpts/s, kNN {extra['f64_knn_qps'] / 1e6:.1f} Mq/s")
+        del d, i, P64
+        t64.free()
+    else:
+        tree.free()
+
+    cpu = None
+    if not args.no_cpu:
+        cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, len(os.sched_getaffinity(0)))
+        log(f"cpu baseline on {cores} host threads ...")
+        res = cpu_baseline(args.cpu_n, args.cpu_q, K, cores)
+        if res:
+            main_impl = "reference" if "reference" in res else "port"
+            r = res[main_impl]
+            cpu = dict(value=r["knn_qps"], unit="queries/s", cores=cores, kind=main_impl,
+                       sample=f"clustered periodic recipe, n={args.cpu_n} float32 points, leaf 16: build, then "
+                              f"kNN k={K} sorted of {args.cpu_q} sampled points (OpenMP over queries)",
+                       build_mpts_s=r["build_mpts_s"])
+            if "port" in res and main_impl != "port":
+                cpu["port_value"] = res["port"]["knn_qps"]
+                cpu["port_build_mpts_s"] = res["port"]["build_mpts_s"]
+            log(f"cpu: {json.dumps(cpu)}")
+
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "knn_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            tr = json.load(f)
+        if tr.get("n") == N and tr.get("k") == K:
+            traffic = tr.get("bytes_per_launch")
+
+    out = {
+        "metric": METRIC,
+        "value": qps,
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: clustered periodic box (seeded numpy PCG64), no dataset",
+        "config": {"workload": "self-query kNN k=32 sorted, clustered periodic box N=2^24 L=100 halo 50%, "
+                               "float32, leaf 16, plain traversal", "n_points": N, "queries": N, "k": K,
+                   "leaf": 16, "boxsize": L},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_knn<float,3,periodic,32>", "bytes_per_query": bq},
+        "cpu_baseline": cpu,
+        "extra": extra,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -28,7 +28,30 @@ int hip_fail(hipError_t e, const char *what, const char *file, int line) {
     return e == hipErrorOutOfMemory ? CKD_ERR_MEMORY : CKD_ERR_HIP;
 }
 
+int workspace(ckd_tree *t, size_t bytes, void **p, hipStream_t s) {
+    if (t->ws_bytes < bytes) {
+        if (t->ws) {
+            CKD_HIP(hipStreamSynchronize(s));
+            (void)hipFree(t->ws);
+            t->ws = nullptr;
+            t->ws_bytes = 0;
+        }
+        const size_t want = bytes + bytes / 4;
+        hipError_t e = hipMalloc(&t->ws, want);
+        if (e != hipSuccess) {
+            t->ws = nullptr;
+            return hip_fail(e, "workspace", __FILE__, __LINE__);
+        }
+        t->ws_bytes = want;
+    }
+    *p = t->ws;
+    return CKD_OK;
+}
+
 static void release(ckd_tree *t) {
+    if (t->ws) (void)hipFree(t->ws);
+    t->ws = nullptr;
+    t->ws_bytes = 0;
     if (t->nodes) (void)hipFree(t->nodes);
     if (t->bbox) (void)hipFree(t->bbox);
     if (t->range) (void)hipFree(t->range);

@@ -205,6 +205,8 @@ struct ckd_tree {
     ckd::Range *range = nullptr;
     void *pts = nullptr;     // T[n * pstride(ndim)]
     void *idx = nullptr;     // int32 / int64 [n]
+    void *ws = nullptr;      // grow-only query workspace (calls on one tree must not overlap)
+    size_t ws_bytes = 0;
 };
 
 namespace ckd {
@@ -221,4 +223,7 @@ TreeView<T> view(const ckd_tree *t) {
     return v;
 }
 int build_tree(ckd_tree *t, const void *points, hipStream_t s);
+// Device workspace of at least `bytes` owned by the tree (grown on demand;
+// growing synchronises `s` before releasing the old buffer).
+int workspace(ckd_tree *t, size_t bytes, void **p, hipStream_t s);
 }  // namespace ckd

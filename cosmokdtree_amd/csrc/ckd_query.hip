@@ -24,6 +24,7 @@ namespace ckd {
 namespace {
 
 constexpr int QT = 256;  // threads per block
+constexpr int64_t ORDER_MIN_QUERIES = 4096;  // below this, queries run in input order
 
 template <typename T, int D>
 __device__ __forceinline__ void load_pt(const T *pts, int64_t p, T (&x)[D]) {
@@ -69,6 +70,46 @@ __device__ __forceinline__ void list_insert(T (&bd)[KMAX], int32_t (&bs)[KMAX], 
     bs[0] = here ? s : bs[0];
 }
 
+// The leaf a point falls in: the nearer child at every level (the first
+// descent of knn_search_recursive_hyperp, cosmokdtree.f90:820-836).
+template <typename T, int D>
+__device__ __forceinline__ int32_t find_leaf(const TreeView<T> &tv, const T (&q)[D]) {
+    int32_t node = 0;
+    for (;;) {
+        const NodeRec<T> nr = tv.nodes[node];
+        if (nr.axis < 0) return node;
+        node = nr.child + (q[nr.axis] < nr.split ? 0 : 1);
+    }
+}
+
+// Query ordering: each query's leaf, and its rank among the queries of
+// that leaf; a counting sort then lists the queries leaf by leaf so a wave
+// works on neighbouring queries.
+template <typename T, int D>
+__global__ __launch_bounds__(QT) void k_locate(TreeView<T> tv, const T *queries, int64_t nq, int32_t *leaf_of,
+                                               int32_t *rank, int64_t *leaf_cnt) {
+    const int64_t qi = (int64_t)blockIdx.x * QT + threadIdx.x;
+    if (qi >= nq) return;
+    T q[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) q[d] = queries[qi * D + d];
+    const int32_t leaf = find_leaf<T, D>(tv, q);
+    leaf_of[qi] = leaf;
+    rank[qi] = (int32_t)atomicAdd((unsigned long long *)&leaf_cnt[leaf], 1ull);
+}
+
+__global__ void k_order(const int32_t *leaf_of, const int32_t *rank, const int64_t *leaf_off, int64_t nq,
+                        int32_t *order) {
+    const int64_t qi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (qi >= nq) return;
+    order[leaf_off[leaf_of[qi]] + rank[qi]] = (int32_t)qi;
+}
+
+__global__ void k_fill_i64(int64_t *p, int64_t n, int64_t v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 template <typename T, int D, bool PER, int KMAX>
 __device__ __forceinline__ void knn_scan_leaf(const TreeView<T> &tv, int32_t leaf, const T (&q)[D], T (&bd)[KMAX],
                                               int32_t (&bs)[KMAX], int64_t &tested) {
@@ -84,22 +125,19 @@ __device__ __forceinline__ void knn_scan_leaf(const TreeView<T> &tv, int32_t lea
 
 template <typename T, int D, bool PER, int KMAX>
 __global__ __launch_bounds__(QT) void k_knn(TreeView<T> tv, const T *queries, int64_t nq, int k, int sorted,
-                                            const float *seed, T *dout, int64_t *iout, int64_t *stats) {
-    const int64_t qi = (int64_t)blockIdx.x * QT + threadIdx.x;
+                                            const float *seed, const int32_t *order, const int32_t *leaf_of,
+                                            T *dout, int64_t *iout, int64_t *stats) {
+    const int64_t t = (int64_t)blockIdx.x * QT + threadIdx.x;
     int64_t visited = 0, tested = 0, reruns = 0;
-    if (qi < nq) {
+    if (t < nq) {
+    const int64_t qi = order ? (int64_t)order[t] : t;
     T q[D];
 #pragma unroll
     for (int d = 0; d < D; d++) q[d] = queries[qi * D + d];
     T bd[KMAX];
     int32_t bs[KMAX];
     // the leaf the query falls in: nearer child at every level
-    int32_t leaf = 0;
-    for (;;) {
-        const NodeRec<T> nr = tv.nodes[leaf];
-        if (nr.axis < 0) break;
-        leaf = nr.child + (q[nr.axis] < nr.split ? 0 : 1);
-    }
+    int32_t leaf = leaf_of ? leaf_of[qi] : find_leaf<T, D>(tv, q);
     T init = T(FLT_MAX);
     if (seed) {
         const T sr = (T)seed[qi];
@@ -188,7 +226,7 @@ __global__ __launch_bounds__(QT) void k_knn(TreeView<T> tv, const T *queries, in
     }
     }
     if (stats) {  // one atomic per wave and counter
-        int64_t nqw = qi < nq ? 1 : 0;
+        int64_t nqw = t < nq ? 1 : 0;
 #pragma unroll
         for (int m = WAVE / 2; m >= 1; m >>= 1) {
             visited += __shfl_xor(visited, m, WAVE);
@@ -452,10 +490,30 @@ int knn_launch(const ckd_tree *t, const T *q, int64_t nq, int k, int sorted, con
     const TreeView<T> tv = view<T>(t);
     const unsigned g = (unsigned)ceil_div(nq, QT);
     if (k <= 64 && t->n < INT32_MAX) {
-        if (k <= 8) k_knn<T, D, PER, 8><<<g, QT, 0, s>>>(tv, q, nq, k, sorted, seed, dist, idx, stats);
-        else if (k <= 16) k_knn<T, D, PER, 16><<<g, QT, 0, s>>>(tv, q, nq, k, sorted, seed, dist, idx, stats);
-        else if (k <= 32) k_knn<T, D, PER, 32><<<g, QT, 0, s>>>(tv, q, nq, k, sorted, seed, dist, idx, stats);
-        else k_knn<T, D, PER, 64><<<g, QT, 0, s>>>(tv, q, nq, k, sorted, seed, dist, idx, stats);
+        const int32_t *order = nullptr, *leaf_of = nullptr;
+        if (nq >= ORDER_MIN_QUERIES && nq < INT32_MAX && t->nnodes > 1) {
+            // workspace: leaf_of, rank, order (int32 x nq), leaf counts and
+            // offsets (int64 x nnodes+1), scan scratch
+            const size_t a = 3 * sizeof(int32_t) * (size_t)nq, b = sizeof(int64_t) * (size_t)(t->nnodes + 1);
+            void *ws = nullptr;
+            int rc = workspace(const_cast<ckd_tree *>(t), a + 2 * b + scan_tmp_bytes(t->nnodes) + 256, &ws, s);
+            if (rc) return rc;
+            int32_t *lo = (int32_t *)ws, *rk = lo + nq, *ord = rk + nq;
+            int64_t *cnt = (int64_t *)((char *)ws + ((a + 255) & ~(size_t)255));
+            int64_t *off = cnt + t->nnodes + 1;
+            void *scr = off + t->nnodes + 1;
+            k_fill_i64<<<(unsigned)ceil_div(t->nnodes, 256), 256, 0, s>>>(cnt, t->nnodes, 0);
+            k_locate<T, D><<<g, QT, 0, s>>>(tv, q, nq, lo, rk, cnt);
+            if ((rc = scan_i64(cnt, off, t->nnodes, false, scr, s))) return rc;
+            k_order<<<(unsigned)ceil_div(nq, 256), 256, 0, s>>>(lo, rk, off, nq, ord);
+            CKD_HIP(hipGetLastError());
+            order = ord;
+            leaf_of = lo;
+        }
+        if (k <= 8) k_knn<T, D, PER, 8><<<g, QT, 0, s>>>(tv, q, nq, k, sorted, seed, order, leaf_of, dist, idx, stats);
+        else if (k <= 16) k_knn<T, D, PER, 16><<<g, QT, 0, s>>>(tv, q, nq, k, sorted, seed, order, leaf_of, dist, idx, stats);
+        else if (k <= 32) k_knn<T, D, PER, 32><<<g, QT, 0, s>>>(tv, q, nq, k, sorted, seed, order, leaf_of, dist, idx, stats);
+        else k_knn<T, D, PER, 64><<<g, QT, 0, s>>>(tv, q, nq, k, sorted, seed, order, leaf_of, dist, idx, stats);
         CKD_HIP(hipGetLastError());
         return CKD_OK;
     }

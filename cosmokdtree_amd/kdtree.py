@@ -173,8 +173,9 @@ def _tree(tree) -> KDTree:
     return t
 
 
-def _build(points, leaf: int, boxsize) -> KDTree:
-    """Build from a torch tensor or array-like; boxsize None = non-periodic."""
+def _build(points, leaf: int, boxsize, keep_points: bool = False) -> KDTree:
+    """Build from a torch tensor or array-like; boxsize None = non-periodic.
+    keep_points keeps the device copy of the input on the handle (.points)."""
     _require_gpu()
     if isinstance(points, torch.Tensor):
         dt = torch.float32 if points.dtype == torch.float32 else torch.float64
@@ -197,7 +198,8 @@ def _build(points, leaf: int, boxsize) -> KDTree:
     tree = KDTree(out.value, n=n, ndim=d, leafsize=int(leaf), torch_dtype=dt,
                   np_dtype=np.float32 if prec == 4 else np.float64, device=pts.device,
                   periodic=boxsize is not None,
-                  boxsize=None if boxsize is None else np.asarray(boxsize, dtype=np.float64))
+                  boxsize=None if boxsize is None else np.asarray(boxsize, dtype=np.float64),
+                  points=pts if keep_points else None)
     _REGISTRY[int(tree)] = tree
     return tree
 

@@ -144,6 +144,40 @@ int ckd_box_fill(const ckd_tree *tree, const void *boxes, int64_t nb, const int6
 int ckd_tree_dump(const ckd_tree *tree, int32_t *axis, void *split, void *lo, void *hi, int32_t *child,
                   int32_t *parent, int64_t *start, int64_t *count, void *points, int64_t *idx);
 
+/*
+ * Learned search-radius proposal (no reference counterpart: it only seeds
+ * the kNN pruning bound through ckd_knn's seed_r, and a short seed falls
+ * back to the plain search).  Network 32 -> 64 -> 64 -> 1 with GELU, bf16
+ * operands on MFMA, fp32 accumulation; parameters are one fp32 vector of
+ * ckd_mlp_nparam() values: W1[32][64], b1[64], W2[64][64], b2[64], w3[64], b3.
+ */
+int ckd_mlp_nparam(void);
+
+/* Point counts of a periodic 3-D tree in a G^3 grid over [-L/2, L/2)^3:
+   grid [dev, G^3 int32], cell (x, y, z) at (z*G + y)*G + x. */
+int ckd_occupancy(const ckd_tree *tree, int32_t G, int32_t *grid, void *stream);
+
+/* Features of nq queries [dev, (nq, 3), prec] into X [dev, (nq, 32) bf16]:
+   (q + L/2)/L per axis, log1p of the 27 neighbouring grid counts (periodic),
+   two zero pads. */
+int ckd_mlp_features(const void *queries, int64_t nq, int32_t prec, const double *boxsize, int32_t G,
+                     const int32_t *grid, uint16_t *X, void *stream);
+
+/* Forward pass: mode 0 writes y = predicted log radius, mode 1 writes
+   scale * exp(y) (a seed radius for ckd_knn).  out [dev, n float]. */
+int ckd_mlp_forward(const float *params, const uint16_t *X, int64_t n, float scale, int32_t mode, float *out,
+                    void *stream);
+
+/* Mean-squared-error gradient over B rows against target [dev, B float]:
+   grad [dev, nparam float] and *loss [dev] (sum of squared errors) are
+   overwritten. */
+int ckd_mlp_grad(const float *params, const uint16_t *X, const float *target, int64_t B, float *grad, float *loss,
+                 void *stream);
+
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8, bias-corrected for
+   `step` >= 1) on params with moment buffers m, v [dev, nparam float]. */
+int ckd_mlp_adam(float *params, const float *grad, float *m, float *v, int64_t step, float lr, void *stream);
+
 /* Text of the last error on this thread ("" if none). */
 const char *ckd_last_error(void);
 

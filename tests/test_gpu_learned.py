@@ -1,0 +1,140 @@
+"""GPU tests of the learned radius proposal.
+
+* features, forward pass, gradients and Adam of the MFMA kernels against
+  the numpy oracle (oracle/mlp_oracle.py), which rounds the same operands
+  to bfloat16; tolerances cover fp32 accumulation order and the last-bit
+  differences of erf/exp between the GPU and numpy;
+* seeded kNN returns exactly the plain result (distances and indices bit
+  for bit) whatever the seeds: trained, zero (every query falls back), tiny,
+  huge;
+* a trained network cuts the nodes visited and points tested per query.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import mlp_oracle as M
+from tests.parity import assert_knn_rows
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def L_():
+    from cosmokdtree_amd import learned
+    return learned
+
+
+@pytest.fixture(scope="module")
+def clustered(ckd):
+    from cosmokdtree_amd.synthetic import clustered_periodic
+    pts = clustered_periodic(1 << 18, 100.0, 0.5, seed=31, dtype=np.float32)
+    tree = ckd.kdtree._build(pts, 16, np.full(3, 100.0), keep_points=True)
+    yield pts, tree
+    tree.free()
+
+
+def bf16_bits_to_f32(x):
+    return (x.astype(np.uint16).astype(np.uint32) << 16).view(np.float32)
+
+
+def test_occupancy_and_features(L_, clustered):
+    pts, tree = clustered
+    grid = L_.occupancy(tree)
+    want = M.occupancy(pts, 100.0)
+    assert np.array_equal(grid.cpu().numpy(), want)
+    q = torch.from_numpy(pts[:5000]).cuda()
+    X = L_.features(q, tree.boxsize, grid).cpu().numpy()
+    f = M.features(pts[:5000].astype(np.float64), 100.0, want)
+    got = bf16_bits_to_f32(X)
+    assert np.array_equal(got, M.bf16(f).astype(np.float32))
+
+
+def test_forward_matches_oracle(L_):
+    rng = np.random.default_rng(0)
+    p = M.init_params(3)
+    p[M.OFF["b1"][0]:M.OFF["b1"][1]] = rng.standard_normal(64) * 0.1
+    p[M.OFF["b3"][0]] = 0.25
+    Xf = rng.standard_normal((1000, 32)).astype(np.float32)
+    Xf[:, 30:] = 0
+    Xb = (np.ascontiguousarray(Xf).view(np.uint32) + 0x7FFF + ((Xf.view(np.uint32) >> 16) & 1)) >> 16
+    X = torch.from_numpy(Xb.astype(np.uint16).view(np.int16)).cuda()
+    model = L_.RadiusModel(params=p)
+    y = model.forward(X).cpu().numpy()
+    yo, _ = M.forward(p, Xf)
+    assert np.allclose(y, yo, rtol=2e-3, atol=2e-3), np.abs(y - yo).max()
+    s = model.seeds(X, margin=1.5).cpu().numpy()
+    assert np.allclose(s, 1.5 * np.exp(yo), rtol=5e-3)
+
+
+def test_gradients_match_oracle(L_):
+    rng = np.random.default_rng(1)
+    p = M.init_params(4)
+    p[M.OFF["b2"][0]:M.OFF["b2"][1]] = rng.standard_normal(64) * 0.1
+    B = 777  # not a multiple of the tile size
+    Xf = rng.standard_normal((B, 32)).astype(np.float32)
+    Xf[:, 30:] = 0
+    Xb = M.bf16(Xf).astype(np.float32)
+    bits = (Xb.view(np.uint32) >> 16).astype(np.uint16).view(np.int16)
+    t = rng.standard_normal(B).astype(np.float32)
+    model = L_.RadiusModel(params=p)
+    loss, g = model.grad_and_loss(torch.from_numpy(bits).cuda(), torch.from_numpy(t).cuda())
+    g = g.cpu().numpy()
+    lo, go = M.loss_and_grad(p, Xb, t)
+    assert abs(float(loss.item()) - lo) <= 1e-3 * lo
+    for name, (a, b) in M.OFF.items():
+        ref = go[a:b]
+        err = np.abs(g[a:b] - ref).max()
+        scale = np.abs(ref).max() + 1e-6
+        assert err <= 2e-2 * scale, (name, err, scale)
+    # the GPU gradient points downhill: a small step lowers the loss
+    p2 = (p - 1e-2 * g / np.abs(g).max()).astype(np.float32)
+    assert M.loss_and_grad(p2, Xb, t)[0] < lo
+
+
+def test_adam_matches_oracle(L_):
+    rng = np.random.default_rng(2)
+    p = M.init_params(5)
+    model = L_.RadiusModel(params=p)
+    g = rng.standard_normal(M.NPARAM).astype(np.float32)
+    model.grad.copy_(torch.from_numpy(g))
+    model.adam(1e-3)
+    po, _, _ = M.adam(p, g, np.zeros_like(p), np.zeros_like(p), 1, 1e-3)
+    assert np.allclose(model.params.cpu().numpy(), po, rtol=1e-6, atol=1e-7)
+
+
+def test_seeded_knn_is_exact(L_, clustered):
+    pts, tree = clustered
+    q = torch.from_numpy(pts[::37]).cuda()
+    d0, i0 = tree.knn(q, 32, sorted=True)
+    nq = q.shape[0]
+    rng = np.random.default_rng(3)
+    for seeds in (np.zeros(nq), np.full(nq, 1e-3), np.full(nq, 1e3),
+                  d0[:, 31].cpu().numpy() * rng.uniform(0.7, 1.3, nq), d0[:, 31].cpu().numpy()):
+        stats = torch.zeros(4, dtype=torch.int64, device="cuda")
+        d1, i1 = tree.knn(q, 32, sorted=True, seed_r=torch.from_numpy(seeds.astype(np.float32)), stats=stats)
+        assert torch.equal(d0, d1)
+        assert_knn_rows(pts, pts[::37], 32, d0.cpu().numpy(), i0.cpu().numpy(), d1.cpu().numpy(),
+                        i1.cpu().numpy(), L=np.full(3, 100.0))
+
+
+def test_trained_network_gains(L_, clustered, ckd):
+    """Train on an independent point set, then compare visits on this one."""
+    from cosmokdtree_amd.synthetic import clustered_periodic
+    train_pts = clustered_periodic(1 << 18, 100.0, 0.5, seed=77, dtype=np.float32)
+    ttree = ckd.kdtree._build(train_pts, 16, np.full(3, 100.0), keep_points=True)
+    model = L_.fit(ttree, 100000, k=32, epochs=3, quantile=0.98)
+    ttree.free()
+    pts, tree = clustered
+    q = torch.from_numpy(pts[:50000]).cuda()
+    s0 = torch.zeros(4, dtype=torch.int64, device="cuda")
+    s1 = torch.zeros(4, dtype=torch.int64, device="cuda")
+    d0, i0 = tree.knn(q, 32, stats=s0)
+    d1, i1 = L_.seeded_knn(tree, model, q, 32, stats=s1)
+    assert torch.equal(d0, d1)
+    s0, s1 = s0.cpu().numpy(), s1.cpu().numpy()
+    fallback = s1[2] / s1[3]
+    print(f"\nplain: {s0[0] / s0[3]:.1f} nodes, {s0[1] / s0[3]:.1f} points per query; "
+          f"seeded: {s1[0] / s1[3]:.1f} nodes, {s1[1] / s1[3]:.1f} points; fallback {fallback:.3f}")
+    assert fallback < 0.1
+    assert s1[1] < 0.9 * s0[1]  # fewer points tested, re-runs included
