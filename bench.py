@@ -69,6 +69,10 @@ def main():
     ap.add_argument("--k", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true", help="skip the host-core baseline")
     ap.add_argument("--no-extra", action="store_true", help="skip the float64 and ball-search extras")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-seed path")
+    ap.add_argument("--train-pairs", type=int, default=2_000_000)
+    ap.add_argument("--epochs", type=int, default=4)
+    ap.add_argument("--quantile", type=float, default=0.98)
     ap.add_argument("--cpu-n", type=int, default=1 << 22)
     ap.add_argument("--cpu-q", type=int, default=1 << 18)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -126,7 +130,7 @@ def main():
         d, i = tree.knn(P, K, sorted=True)
     del d, i
     torch.cuda.synchronize()
-    stats = torch.zeros(4, dtype=torch.int64, device=dev)
+    stats = torch.zeros(5, dtype=torch.int64, device=dev)
     d, i = tree.knn(P[:1 << 20], K, sorted=True, stats=stats)  # visit counts on a 1M sample (untimed)
     st = stats.cpu().numpy()
     del d, i
@@ -143,12 +147,57 @@ def main():
     log(f"kNN k={K}: {ms_step:.2f} ms/step, {qps / 1e6:.1f} Mq/s")
     s_c, s_i = 4, (8 if info.idx_bits == 64 else 4)
     bq = knn_bytes_per_query(3, s_c, s_i, K, N, info.nnodes, N)
+    plain_qps, plain_ms = qps, ms_step
+    d_plain = d
+    del i
+
+    # ---- learned seed: trained on an independent point set of the same recipe
+    learned_info = None
+    if not args.no_learned:
+        from cosmokdtree_amd import learned
+        from cosmokdtree_amd.synthetic import SEED_TRAIN
+        t0 = time.time()
+        tpts = torch.from_numpy(clustered_periodic(N, L, 0.5, seed=SEED_TRAIN, dtype=np.float32)).to(dev)
+        ttree = kdtree._build(tpts, 16, box, keep_points=True)
+        model = learned.fit(ttree, args.train_pairs, k=K, epochs=args.epochs, quantile=args.quantile, log=log)
+        ttree.free()
+        del tpts
+        grid = learned.occupancy(tree)
+        train_s = time.time() - t0
+        s1 = torch.zeros(5, dtype=torch.int64, device=dev)
+        learned.seeded_knn(tree, model, P[:1 << 20], K, grid=grid, stats=s1)
+        s1 = s1.cpu().numpy()
+        for _ in range(args.warmup):
+            d, i = learned.seeded_knn(tree, model, P, K, grid=grid)
+        del d, i
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(args.steps):
+            d, i = learned.seeded_knn(tree, model, P, K, grid=grid)
+        e1.record()
+        torch.cuda.synchronize()
+        seeded_ms = e0.elapsed_time(e1) / args.steps
+        exact = bool(torch.equal(d, d_plain))
+        learned_info = dict(seeded_qps=N / (seeded_ms / 1e3), seeded_ms_per_step=seeded_ms,
+                            speedup_vs_plain=plain_ms / seeded_ms, distances_identical=exact,
+                            fallback_rate=float(s1[2] / max(s1[3], 1)),
+                            nodes_visited_per_query=float(s1[0] / max(s1[3], 1)),
+                            points_tested_per_query=float(s1[1] / max(s1[3], 1)),
+                            insertions_per_query=float(s1[4] / max(s1[3], 1)), margin=model.margin,
+                            train_pairs=args.train_pairs, train_seconds=train_s)
+        log(f"learned seed: {seeded_ms:.2f} ms/step ({N / seeded_ms / 1e3:.1f} Mq/s), "
+            f"x{plain_ms / seeded_ms:.2f} vs plain, exact={exact}, fallback {learned_info['fallback_rate']:.4f}")
+        del d, i
+    del d_plain
+    if learned_info and learned_info["distances_identical"]:
+        qps, ms_step = learned_info["seeded_qps"], learned_info["seeded_ms_per_step"]
     achieved = bq * N / (ms_step / 1e3) / 1e9
-    del d, i
 
     extra = dict(build_ms=build_ms, build_mpts_s=N / build_ms / 1e3, tree_nodes=int(info.nnodes),
                  tree_depth=int(info.depth), nodes_visited_per_query=float(st[0] / max(st[3], 1)),
                  points_tested_per_query=float(st[1] / max(st[3], 1)),
+                 insertions_per_query=float(st[4] / max(st[3], 1)),
+                 plain_qps=plain_qps, plain_ms_per_step=plain_ms, learned=learned_info,
                  build_bytes_per_point=build_bytes_per_point(3, 4, s_i, N, 16))
     extra["build_roofline_frac"] = extra["build_bytes_per_point"] * N / (build_ms / 1e3) / 1e9 / HBM_PEAK_GBS
 
@@ -227,8 +276,10 @@ def main():
         "dtype": "f32",
         "data": "synthetic: clustered periodic box (seeded numpy PCG64), no dataset",
         "config": {"workload": "self-query kNN k=32 sorted, clustered periodic box N=2^24 L=100 halo 50%, "
-                               "float32, leaf 16, plain traversal", "n_points": N, "queries": N, "k": K,
-                   "leaf": 16, "boxsize": L},
+                               "float32, leaf 16; value = learned-seed path (features + MLP + seeded "
+                               "traversal) when its distances equal the plain path's, else the plain path",
+                   "n_points": N, "queries": N, "k": K, "leaf": 16, "boxsize": L,
+                   "path": "learned-seed" if learned_info and learned_info["distances_identical"] else "plain"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_knn<float,3,periodic,32>", "bytes_per_query": bq},

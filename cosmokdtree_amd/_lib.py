@@ -25,7 +25,7 @@ CKD_EMPTY_SLOT = -2
 CKD_MAX_DIM = 8
 
 EXPORTS = (
-    "ckd_build", "ckd_free", "ckd_get_info", "ckd_knn", "ckd_ball_count", "ckd_ball_fill",
+    "ckd_build", "ckd_build_forest", "ckd_free", "ckd_get_info", "ckd_knn", "ckd_ball_count", "ckd_ball_fill",
     "ckd_box_count", "ckd_box_fill", "ckd_tree_dump", "ckd_last_error", "ckd_version",
 )
 
@@ -35,6 +35,7 @@ class TreeInfo(C.Structure):
         ("n", C.c_int64), ("ndim", C.c_int32), ("prec", C.c_int32), ("leafsize", C.c_int32),
         ("periodic", C.c_int32), ("idx_bits", C.c_int32), ("depth", C.c_int32), ("nnodes", C.c_int64),
         ("nleaves", C.c_int64), ("boxsize", C.c_double * CKD_MAX_DIM), ("build_ms", C.c_double),
+        ("nsets", C.c_int64),
     ]
 
 
@@ -45,6 +46,7 @@ def _load():
     lib = C.CDLL(LIB_PATH)
     vp, i64, i32, dbl = C.c_void_p, C.c_int64, C.c_int32, C.c_double
     lib.ckd_build.argtypes = [vp, i64, i32, i32, i32, vp, i32, vp, C.POINTER(vp)]
+    lib.ckd_build_forest.argtypes = [vp, vp, i64, i32, i32, i32, vp, i32, vp, C.POINTER(vp)]
     lib.ckd_free.argtypes = [vp]
     lib.ckd_get_info.argtypes = [vp, C.POINTER(TreeInfo)]
     lib.ckd_knn.argtypes = [vp, vp, i64, i32, i32, vp, vp, vp, vp, vp]

@@ -1,0 +1,165 @@
+"""Batched queries over many independent point sets (a forest).
+
+    forest = build_forest([pts_0, pts_1, ...], leaf=16, boxsize=[10] * 3)
+    plan = BatchPlan(forest, queries_per_set, k_per_set, radius_per_set)
+    out = plan.run(queries)          # eager
+    plan.capture(queries)            # record the same launches in a hipGraph
+    out = plan.replay()              # replay the graph
+
+All sets are built in one level-synchronous pass (ckd_build_forest) and
+all queries of all sets, with their own k and ball radius, are answered by
+persistent kernels that drain one device work queue (ckd_batch_*).  Each
+set's results equal those of the single-tree calls on that set; indices are
+relative to the set.  No reference counterpart: the reference answers one
+tree per call (src/cosmokdtree/kdtree.py).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from ._lib import check, lib
+from .kdtree import KDTree, _REGISTRY, _require_gpu, _stream, _vp
+
+_vp_t = C.c_void_p
+lib.ckd_batch_plan.argtypes = [_vp_t, _vp_t, _vp_t, _vp_t, C.c_int32, C.c_int32, C.c_int32, C.c_int64,
+                               C.POINTER(_vp_t), C.POINTER(C.c_int64)]
+lib.ckd_batch_run.argtypes = [_vp_t] * 8
+lib.ckd_batch_status.argtypes = [_vp_t, _vp_t, C.POINTER(C.c_int64), C.POINTER(C.c_int32), _vp_t]
+lib.ckd_batch_free.argtypes = [_vp_t]
+for _n in ("ckd_batch_plan", "ckd_batch_run", "ckd_batch_status", "ckd_batch_free"):
+    getattr(lib, _n).restype = C.c_int
+
+
+def build_forest(sets, leaf: int = 16, boxsize=None, dtype=None) -> KDTree:
+    """One tree per point set, built in one pass.  `sets` is a list of (n_i, 3)
+    arrays/tensors, or a pair (points (N, 3), offsets (nsets+1))."""
+    _require_gpu()
+    if isinstance(sets, tuple) and len(sets) == 2:
+        pts, off = sets
+        off = np.asarray(off, dtype=np.int64)
+        pts = torch.as_tensor(pts)
+    else:
+        arrs = [torch.as_tensor(np.asarray(s) if not isinstance(s, torch.Tensor) else s) for s in sets]
+        off = np.zeros(len(arrs) + 1, dtype=np.int64)
+        np.cumsum([a.shape[0] for a in arrs], out=off[1:])
+        pts = torch.cat([a.to("cuda") for a in arrs]) if arrs else torch.empty((0, 3))
+    dt = dtype or (torch.float32 if pts.dtype == torch.float32 else torch.float64)
+    pts = pts.to(device="cuda", dtype=dt).contiguous()
+    n, d = pts.shape
+    bs = None if boxsize is None else (C.c_double * d)(*[float(x) for x in boxsize])
+    out = C.c_void_p()
+    prec = 4 if dt == torch.float32 else 8
+    check(lib.ckd_build_forest(_vp(pts), off.ctypes.data_as(C.c_void_p), len(off) - 1, d, prec, int(leaf), bs, 0,
+                               _stream(), C.byref(out)), "build_forest")
+    tree = KDTree(out.value, n=n, ndim=d, leafsize=int(leaf), torch_dtype=dt,
+                  np_dtype=np.float32 if prec == 4 else np.float64, device=pts.device,
+                  periodic=boxsize is not None,
+                  boxsize=None if boxsize is None else np.asarray(boxsize, dtype=np.float64), points=None,
+                  set_offsets=off)
+    _REGISTRY[int(tree)] = tree
+    return tree
+
+
+class BatchPlan:
+    """Work queue and output layout for one batched query workload."""
+
+    def __init__(self, forest: KDTree, nq_per_set, k_per_set, radius_per_set=None, sorted: bool = True,
+                 threads: int = 256, blocks: int = 0, ball_capacity: int | None = None):
+        self.forest = forest
+        nq = np.asarray(nq_per_set, dtype=np.int64)
+        self.qoff = np.zeros(len(nq) + 1, dtype=np.int64)
+        np.cumsum(nq, out=self.qoff[1:])
+        self.k = np.ascontiguousarray(np.asarray(k_per_set, dtype=np.int32))
+        self.radius = None if radius_per_set is None else np.ascontiguousarray(
+            np.asarray(radius_per_set, dtype=np.float64))
+        self.nq = int(self.qoff[-1])
+        self.sorted = bool(sorted)
+        self.threads, self.blocks = int(threads), int(blocks)
+        self.ball_capacity = ball_capacity
+        self._plan = None
+        self._graph = None
+        self._make(ball_capacity or 0)
+
+    def _make(self, capacity):
+        if self._plan:
+            lib.ckd_batch_free(self._plan)
+        plan = C.c_void_p()
+        tot = C.c_int64(0)
+        check(lib.ckd_batch_plan(self.forest._ptr, self.qoff.ctypes.data_as(C.c_void_p),
+                                 self.k.ctypes.data_as(C.c_void_p),
+                                 None if self.radius is None else self.radius.ctypes.data_as(C.c_void_p),
+                                 int(self.sorted), self.threads, self.blocks, int(capacity), C.byref(plan),
+                                 C.byref(tot)), "batch_plan")
+        self._plan = plan
+        self.knn_total = tot.value
+        self.capacity = int(capacity)
+        dev, dt = self.forest.device, self.forest.torch_dtype
+        self.knn_dist = torch.empty(max(self.knn_total, 1), dtype=dt, device=dev)
+        self.knn_idx = torch.empty(max(self.knn_total, 1), dtype=torch.int64, device=dev)
+        self.ball_off = torch.empty(self.nq + 1, dtype=torch.int64, device=dev)
+        self.ball_dist = torch.empty(max(self.capacity, 1), dtype=dt, device=dev)
+        self.ball_idx = torch.empty(max(self.capacity, 1), dtype=torch.int64, device=dev)
+
+    def __del__(self):
+        if getattr(self, "_plan", None):
+            lib.ckd_batch_free(self._plan)
+            self._plan = None
+
+    def launch(self, queries: torch.Tensor):
+        """Enqueue the batched kernels on the current stream (capturable)."""
+        self._q = queries
+        check(lib.ckd_batch_run(self._plan, _vp(queries), _vp(self.knn_dist), _vp(self.knn_idx),
+                                _vp(self.ball_off), _vp(self.ball_dist), _vp(self.ball_idx), _stream()),
+              "batch_run")
+
+    def status(self):
+        tot, ovf = C.c_int64(0), C.c_int32(0)
+        check(lib.ckd_batch_status(self._plan, _vp(self.ball_off), C.byref(tot), C.byref(ovf), _stream()),
+              "batch_status")
+        return tot.value, bool(ovf.value)
+
+    def run(self, queries: torch.Tensor):
+        """Eager run; grows the ball capacity and reruns once if it was short."""
+        queries = queries.to(device=self.forest.device, dtype=self.forest.torch_dtype).contiguous()
+        self.launch(queries)
+        if self.radius is not None:
+            tot, ovf = self.status()
+            if ovf:
+                self._make(tot)
+                self.launch(queries)
+        return self.results()
+
+    def results(self):
+        tot = int(self.ball_off[-1].item()) if self.radius is not None else 0
+        return dict(knn_dist=self.knn_dist[:self.knn_total], knn_idx=self.knn_idx[:self.knn_total],
+                    ball_off=self.ball_off, ball_dist=self.ball_dist[:tot], ball_idx=self.ball_idx[:tot])
+
+    def capture(self, queries: torch.Tensor):
+        """Record the batched launches in a hipGraph (run eagerly first so the
+        ball capacity is known)."""
+        queries = queries.to(device=self.forest.device, dtype=self.forest.torch_dtype).contiguous()
+        self.run(queries)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            with torch.cuda.graph(g, stream=s):
+                self.launch(queries)
+        torch.cuda.current_stream().wait_stream(s)
+        self._graph = g
+        return g
+
+    def replay(self):
+        self._graph.replay()
+        return self.results()
+
+    def knn_rows(self, r: int):
+        """(dist, idx) of set r as (nq_r, k_r) views."""
+        k = int(self.k[r])
+        nq = int(self.qoff[r + 1] - self.qoff[r])
+        o = int(np.sum((self.qoff[1:r + 1] - self.qoff[:r]) * self.k[:r]))
+        return self.knn_dist[o:o + nq * k].view(nq, k), self.knn_idx[o:o + nq * k].view(nq, k)

@@ -49,6 +49,8 @@ int workspace(ckd_tree *t, size_t bytes, void **p, hipStream_t s) {
 }
 
 static void release(ckd_tree *t) {
+    if (t->set_off) (void)hipFree(t->set_off);
+    t->set_off = nullptr;
     if (t->ws) (void)hipFree(t->ws);
     t->ws = nullptr;
     t->ws_bytes = 0;
@@ -68,7 +70,7 @@ static int counts_to_offsets(int64_t *offsets, int64_t n, int64_t *total, hipStr
     void *scr = nullptr;
     CKD_HIP(hipMalloc(&tmp, sizeof(int64_t) * (n > 0 ? n : 1)));
     CKD_HIP(hipMalloc(&scr, scan_tmp_bytes(n)));
-    if (n > 0) CKD_HIP(hipMemcpyAsync(tmp, offsets, sizeof(int64_t) * n, hipMemcpyDeviceToDevice, s));
+    if (n > 0) CKD_HIP(hipMemcpyAsync(tmp, offsets + 1, sizeof(int64_t) * n, hipMemcpyDeviceToDevice, s));
     int rc = scan_i64(tmp, offsets, n, true, scr, s);
     int64_t tot = 0;
     if (!rc) {
@@ -92,8 +94,9 @@ const char *ckd_last_error(void) { return g_err; }
 
 const char *ckd_version(void) { return "cosmokdtree_amd 0.1 (gfx950)"; }
 
-int ckd_build(const void *points, int64_t n, int32_t ndim, int32_t prec, int32_t leaf, const double *boxsize,
-              int32_t idx_bits, void *stream, ckd_tree **out) {
+static int build_common(const void *points, int64_t n, const int64_t *set_offsets, int64_t nsets, int32_t ndim,
+                        int32_t prec, int32_t leaf, const double *boxsize, int32_t idx_bits, void *stream,
+                        ckd_tree **out) {
     g_err[0] = 0;
     if (!out) {
         set_error("out must not be NULL");
@@ -116,6 +119,21 @@ int ckd_build(const void *points, int64_t n, int32_t ndim, int32_t prec, int32_t
         set_error("at least one point is required");
         return CKD_ERR_ARG;
     }
+    if (nsets < 1 || nsets > INT32_MAX / 4) {
+        set_error("invalid number of point sets %lld", (long long)nsets);
+        return CKD_ERR_ARG;
+    }
+    if (set_offsets) {
+        if (set_offsets[0] != 0 || set_offsets[nsets] != n) {
+            set_error("set offsets must start at 0 and end at n");
+            return CKD_ERR_ARG;
+        }
+        for (int64_t r = 0; r < nsets; r++)
+            if (set_offsets[r + 1] <= set_offsets[r]) {
+                set_error("point set %lld is empty or offsets decrease", (long long)r);
+                return CKD_ERR_ARG;
+            }
+    }
     if (idx_bits == 0) idx_bits = n < INT32_MAX ? 32 : 64;
     if (idx_bits != 32 && idx_bits != 64) {
         set_error("idx_bits must be 0, 32 or 64");
@@ -131,6 +149,7 @@ int ckd_build(const void *points, int64_t n, int32_t ndim, int32_t prec, int32_t
     t->prec = prec;
     t->leafsize = leaf;
     t->idx_bits = idx_bits;
+    t->nsets = nsets;
     if (boxsize) {
         t->periodic = 1;
         for (int d = 0; d < ndim; d++) {
@@ -143,7 +162,17 @@ int ckd_build(const void *points, int64_t n, int32_t ndim, int32_t prec, int32_t
         }
     }
     (void)hipGetDevice(&t->device);
-    int rc = build_tree(t, points, (hipStream_t)stream);
+    hipStream_t s = (hipStream_t)stream;
+    hipError_t e = hipMalloc(&t->set_off, sizeof(int64_t) * (nsets + 1));
+    if (e == hipSuccess) {
+        if (set_offsets) {
+            e = hipMemcpy(t->set_off, set_offsets, sizeof(int64_t) * (nsets + 1), hipMemcpyHostToDevice);
+        } else {
+            const int64_t two[2] = {0, n};
+            e = hipMemcpy(t->set_off, two, sizeof(two), hipMemcpyHostToDevice);
+        }
+    }
+    int rc = e == hipSuccess ? build_tree(t, points, s) : hip_fail(e, "set offsets", __FILE__, __LINE__);
     if (rc) {
         release(t);
         delete t;
@@ -151,6 +180,21 @@ int ckd_build(const void *points, int64_t n, int32_t ndim, int32_t prec, int32_t
     }
     *out = t;
     return CKD_OK;
+}
+
+int ckd_build(const void *points, int64_t n, int32_t ndim, int32_t prec, int32_t leaf, const double *boxsize,
+              int32_t idx_bits, void *stream, ckd_tree **out) {
+    return build_common(points, n, nullptr, 1, ndim, prec, leaf, boxsize, idx_bits, stream, out);
+}
+
+int ckd_build_forest(const void *points, const int64_t *set_offsets, int64_t nsets, int32_t ndim, int32_t prec,
+                     int32_t leaf, const double *boxsize, int32_t idx_bits, void *stream, ckd_tree **out) {
+    if (!set_offsets || nsets < 1) {
+        set_error("set_offsets must not be NULL and nsets must be positive");
+        return CKD_ERR_ARG;
+    }
+    return build_common(points, set_offsets[nsets], set_offsets, nsets, ndim, prec, leaf, boxsize, idx_bits, stream,
+                        out);
 }
 
 int ckd_free(ckd_tree *t) {
@@ -177,6 +221,7 @@ int ckd_get_info(const ckd_tree *t, ckd_tree_info *info) {
     info->nleaves = t->nleaves;
     for (int d = 0; d < CKD_MAX_DIM; d++) info->boxsize[d] = t->L[d];
     info->build_ms = t->build_ms;
+    info->nsets = t->nsets;
     return CKD_OK;
 }
 

@@ -348,15 +348,47 @@ __global__ void k_copy_leaves(const int64_t *child_active, int64_t m, int32_t ne
     FI[p] = I[p];
 }
 
-// Root that is already a leaf (n <= leafsize): bounds + copy.
+// Roots of all point sets (node r = set r, points [set_off[r], set_off[r+1])).
+// A root is active when it holds more than leafsize points.
+template <typename T, int D>
+__global__ void k_init_roots(NodeRec<T> *nodes, T *bbox, Range *range, const int64_t *set_off, int64_t nsets,
+                             TreeView<T> tv, int periodic, int leafsize, int64_t *root_active) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nsets) return;
+    NodeRec<T> nr;
+    nr.split = T(0);
+    nr.axis = -1;
+    nr.child = -1;
+    nr.parent = -1;
+    nodes[r] = nr;
+    Range rg;
+    rg.start = set_off[r];
+    rg.count = set_off[r + 1] - set_off[r];
+    range[r] = rg;
+    root_active[r] = rg.count > leafsize ? 1 : 0;
+    if (periodic) {  // cosmokdtree.f90:218-224
+        for (int d = 0; d < D; d++) {
+            bbox[r * 2 * D + d] = -tv.L[d] / T(2);
+            bbox[r * 2 * D + D + d] = tv.L[d] / T(2);
+        }
+    }
+}
+
+// Roots that are leaves already (at most leafsize points): cell bounds
+// from the data when non-periodic (cosmokdtree.f90:226-229), and the copy
+// into the leaf buckets.  One block per root.
 template <typename T, int D, typename IdxT>
-__global__ __launch_bounds__(BT) void k_root_leaf(const T *P, const IdxT *I, int64_t n, NodeRec<T> *nodes, T *bbox,
-                                                  Range *range, int root_from_data, T *F, IdxT *FI) {
+__global__ __launch_bounds__(BT) void k_root_leaves(const T *P, const IdxT *I, int64_t n, const int64_t *set_off,
+                                                    const int64_t *root_active, T *bbox, int root_from_data, T *F,
+                                                    IdxT *FI) {
     __shared__ Ext<T> lds[BT / WAVE][D];
+    const int64_t r = blockIdx.x;
+    if (root_active[r]) return;
+    const int64_t b = set_off[r], e_ = set_off[r + 1];
     Ext<T> e[D];
 #pragma unroll
     for (int d = 0; d < D; d++) e[d] = ext_empty<T>();
-    for (int64_t p = threadIdx.x; p < n; p += BT)
+    for (int64_t p = b + threadIdx.x; p < e_; p += BT)
 #pragma unroll
         for (int d = 0; d < D; d++) {
             T x = P[(int64_t)d * n + p];
@@ -365,51 +397,19 @@ __global__ __launch_bounds__(BT) void k_root_leaf(const T *P, const IdxT *I, int
         }
     block_reduce_ext<T, D>(e, lds);
     constexpr int PS = pstride(D);
-    for (int64_t p = threadIdx.x; p < n; p += BT) {
+    for (int64_t p = b + threadIdx.x; p < e_; p += BT) {
 #pragma unroll
         for (int d = 0; d < D; d++) F[p * PS + d] = P[(int64_t)d * n + p];
 #pragma unroll
         for (int d = D; d < PS; d++) F[p * PS + d] = T(0);
         FI[p] = I[p];
     }
-    if (threadIdx.x == 0) {
-        if (root_from_data)
+    if (threadIdx.x == 0 && root_from_data)
 #pragma unroll
-            for (int d = 0; d < D; d++) {
-                bbox[d] = e[d].mn;
-                bbox[D + d] = e[d].mx;
-            }
-        NodeRec<T> nr;
-        nr.split = T(0);
-        nr.axis = -1;
-        nr.child = -1;
-        nr.parent = -1;
-        nodes[0] = nr;
-        Range r;
-        r.start = 0;
-        r.count = n;
-        range[0] = r;
-    }
-}
-
-template <typename T, int D>
-__global__ void k_init_root(NodeRec<T> *nodes, T *bbox, Range *range, int64_t n, TreeView<T> tv, int periodic) {
-    NodeRec<T> nr;
-    nr.split = T(0);
-    nr.axis = -1;
-    nr.child = -1;
-    nr.parent = -1;
-    nodes[0] = nr;
-    Range r;
-    r.start = 0;
-    r.count = n;
-    range[0] = r;
-    if (periodic) {  // cosmokdtree.f90:218-224
         for (int d = 0; d < D; d++) {
-            bbox[d] = -tv.L[d] / T(2);
-            bbox[D + d] = tv.L[d] / T(2);
+            bbox[r * 2 * D + d] = e[d].mn;
+            bbox[r * 2 * D + D + d] = e[d].mx;
         }
-    }
 }
 
 // Periodic range check (cosmokdtree.f90:176-190): counts points outside.
@@ -481,8 +481,10 @@ int build_impl(ckd_tree *t, const T *in, hipStream_t s) {
     CKD_HIP(hipMalloc(&t->pts, sizeof(T) * PS * n));
     CKD_HIP(hipMalloc(&t->idx, sizeof(IdxT) * n));
 
+    const int64_t nsets_ = t->nsets;
     int64_t cap = 4 * n / (leaf > 1 ? leaf : 1) + 64;
     if (cap > 2 * n + 1) cap = 2 * n + 1;
+    if (cap < 2 * nsets_ + 64) cap = 2 * nsets_ + 64;
     auto alloc_nodes = [&](int64_t c) -> int {
         void *nn = nullptr, *nb = nullptr, *nr = nullptr;
         CKD_HIP(hipMalloc(&nn, sizeof(NodeRec<T>) * c));
@@ -508,18 +510,11 @@ int build_impl(ckd_tree *t, const T *in, hipStream_t s) {
 
     k_transpose_in<T, D><<<(unsigned)ceil_div(n, BT), BT, 0, s>>>(in, n, P.as<T>(), I.p, sizeof(IdxT) == 8);
     CKD_HIP(hipGetLastError());
-    k_init_root<T, D><<<1, 1, 0, s>>>((NodeRec<T> *)t->nodes, (T *)t->bbox, t->range, n, tvL, t->periodic);
-    t->nnodes = 1;
+    const int64_t nsets = t->nsets;
     t->depth = 0;
-
-    if (n <= leaf) {
-        k_root_leaf<T, D, IdxT><<<1, BT, 0, s>>>(P.as<T>(), I.as<IdxT>(), n, (NodeRec<T> *)t->nodes, (T *)t->bbox,
-                                                 t->range, !t->periodic, (T *)t->pts, (IdxT *)t->idx);
-        CKD_HIP(hipGetLastError());
-        t->nleaves = 1;
-    } else {
+    {
         // per-level work arrays, sized for the largest possible level
-        const int64_t nact_max = n / (leaf + 1) + 1;
+        const int64_t nact_max = (n / (leaf + 1) + 1 > nsets ? n / (leaf + 1) + 1 : nsets);
         const int64_t nchunk_max = n / CH + nact_max + 1;
         DevBuf act, act2, nch, chunk_base, chunks, cstat, dec, cleft, cleft_off, child_active, child_pos, scratch,
             dev_nact;
@@ -533,16 +528,31 @@ int build_impl(ckd_tree *t, const T *in, hipStream_t s) {
             (rc = child_pos.alloc(sizeof(int64_t) * (2 * nact_max + 1))) ||
             (rc = scratch.alloc(scan_tmp_bytes(2 * nact_max + nchunk_max + 2))))
             return rc;
-        int32_t zero = 0;
-        CKD_HIP(hipMemcpyAsync(act.p, &zero, sizeof(int32_t), hipMemcpyHostToDevice, s));
-        int64_t nact = 1;
-        int32_t next_id = 1;
         T *Pc = P.as<T>(), *Pn = P2.as<T>();
         IdxT *Ic = I.as<IdxT>(), *In = I2.as<IdxT>();
         int32_t *actc = act.as<int32_t>(), *actn = act2.as<int32_t>();
-        int64_t nleaves = 0;
         int64_t *h_nact = nullptr;
         CKD_HIP(hipHostMalloc((void **)&h_nact, sizeof(int64_t), hipHostMallocDefault));
+        // roots: one per point set; the active ones start the level loop
+        k_init_roots<T, D><<<(unsigned)ceil_div(nsets, 256), 256, 0, s>>>(
+            (NodeRec<T> *)t->nodes, (T *)t->bbox, t->range, t->set_off, nsets, tvL, t->periodic, leaf,
+            child_active.as<int64_t>());
+        k_root_leaves<T, D, IdxT><<<(unsigned)nsets, BT, 0, s>>>(Pc, Ic, n, t->set_off, child_active.as<int64_t>(),
+                                                                 (T *)t->bbox, !t->periodic, (T *)t->pts,
+                                                                 (IdxT *)t->idx);
+        if ((rc = scan_i64(child_active.as<int64_t>(), child_pos.as<int64_t>(), nsets, true, scratch.p, s))) {
+            (void)hipHostFree(h_nact);
+            return rc;
+        }
+        k_compact_active<<<(unsigned)ceil_div(nsets, 256), 256, 0, s>>>(
+            child_active.as<int64_t>(), child_pos.as<int64_t>(), nsets, 0, actc);
+        CKD_HIP(hipGetLastError());
+        CKD_HIP(hipMemcpyAsync(h_nact, child_pos.as<int64_t>() + nsets, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        CKD_HIP(hipStreamSynchronize(s));
+        int64_t nact = *h_nact;
+        int32_t next_id = (int32_t)nsets;
+        t->nnodes = nsets;
+        int64_t nleaves = nsets - nact;
         int level = 0;
         while (nact > 0) {
             if ((int64_t)next_id + 2 * nact > cap) {

@@ -205,6 +205,8 @@ struct ckd_tree {
     ckd::Range *range = nullptr;
     void *pts = nullptr;     // T[n * pstride(ndim)]
     void *idx = nullptr;     // int32 / int64 [n]
+    int64_t nsets = 1;        // independent point sets (a forest); root of set r is node r
+    int64_t *set_off = nullptr;  // device [nsets + 1]: set r holds input rows [set_off[r], set_off[r+1])
     void *ws = nullptr;      // grow-only query workspace (calls on one tree must not overlap)
     size_t ws_bytes = 0;
 };

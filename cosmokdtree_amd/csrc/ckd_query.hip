@@ -1,23 +1,7 @@
-// ckd_query.hip -- kNN, ball and box queries over a built tree.
-//
-// One thread per query.  Traversals are stack-free: a node's parent
-// pointer replaces the recursion stack of the reference, so any tree depth
-// (sliding-midpoint trees can be very deep on duplicates) is handled
-// without a stack bound.
-//
-// kNN keeps the k best candidates in registers as an ascending list
-// (distance, leaf row) of static length KMAX >= k: the k live entries are
-// the top k slots, the slots below are -inf and inert, so the pruning bound
-// is always slot KMAX-1.  A candidate is accepted only when strictly closer
-// than that bound, as in cosmokdtree.f90:658,808; the list starts filled
-// with FLT_MAX (HUGE(0.0), cosmokdtree.f90:517) or with the squared seed
-// radius.  The visit order is depth-first, nearer child first (the order
-// of knn_search_recursive_hyperp, cosmokdtree.f90:820-846), with every
-// subtree pruned by its cell's minimum distance (bbox_mindist_kd,
-// cosmokdtree.f90:1525-1546; minimum image when periodic).  Pruning never
-// drops a candidate that could be accepted, so the accepted set equals the
-// reference's up to the choice among points tied at the k-th distance.
-#include "ckd_internal.h"
+// ckd_query.hip -- single-tree kNN, ball and box kernels (one thread per
+// query) and their host dispatch; the traversals themselves are in
+// ckd_traverse.h.
+#include "ckd_traverse.h"
 
 namespace ckd {
 
@@ -25,62 +9,6 @@ namespace {
 
 constexpr int QT = 256;  // threads per block
 constexpr int64_t ORDER_MIN_QUERIES = 4096;  // below this, queries run in input order
-
-template <typename T, int D>
-__device__ __forceinline__ void load_pt(const T *pts, int64_t p, T (&x)[D]) {
-    constexpr int PS = pstride(D);
-    const T *src = pts + p * PS;
-    if constexpr (sizeof(T) == 4 && PS == 4) {
-        float4 v = *(const float4 *)src;
-        x[0] = v.x;
-        if (D > 1) x[1] = v.y;
-        if (D > 2) x[2] = v.z;
-        if (D > 3) x[3] = v.w;
-    } else {
-#pragma unroll
-        for (int d = 0; d < D; d++) x[d] = src[d];
-    }
-}
-
-template <typename T, int D>
-__device__ __forceinline__ void load_box(const T *bbox, int32_t node, T (&b)[2 * D]) {
-    const T *src = bbox + (int64_t)node * 2 * D;
-#pragma unroll
-    for (int d = 0; d < 2 * D; d++) b[d] = src[d];
-}
-
-template <typename T, int D>
-__device__ __forceinline__ T node_mindist(const TreeView<T> &tv, int32_t node, const T (&q)[D], bool per) {
-    T b[2 * D];
-    load_box<T, D>(tv.bbox, node, b);
-    return per ? mindist2<T, D, true>(b, q, tv.L) : mindist2<T, D, false>(b, q, tv.L);
-}
-
-template <typename T, int KMAX>
-__device__ __forceinline__ void list_insert(T (&bd)[KMAX], int32_t (&bs)[KMAX], T d, int32_t s) {
-#pragma unroll
-    for (int j = KMAX - 1; j > 0; --j) {
-        const bool sh = d < bd[j - 1];
-        const bool here = !sh && d < bd[j];
-        bd[j] = sh ? bd[j - 1] : (here ? d : bd[j]);
-        bs[j] = sh ? bs[j - 1] : (here ? s : bs[j]);
-    }
-    const bool here = d < bd[0];
-    bd[0] = here ? d : bd[0];
-    bs[0] = here ? s : bs[0];
-}
-
-// The leaf a point falls in: the nearer child at every level (the first
-// descent of knn_search_recursive_hyperp, cosmokdtree.f90:820-836).
-template <typename T, int D>
-__device__ __forceinline__ int32_t find_leaf(const TreeView<T> &tv, const T (&q)[D]) {
-    int32_t node = 0;
-    for (;;) {
-        const NodeRec<T> nr = tv.nodes[node];
-        if (nr.axis < 0) return node;
-        node = nr.child + (q[nr.axis] < nr.split ? 0 : 1);
-    }
-}
 
 // Query ordering: each query's leaf, and its rank among the queries of
 // that leaf; a counting sort then lists the queries leaf by leaf so a wave
@@ -93,7 +21,7 @@ __global__ __launch_bounds__(QT) void k_locate(TreeView<T> tv, const T *queries,
     T q[D];
 #pragma unroll
     for (int d = 0; d < D; d++) q[d] = queries[qi * D + d];
-    const int32_t leaf = find_leaf<T, D>(tv, q);
+    const int32_t leaf = find_leaf<T, D>(tv, q, 0);
     leaf_of[qi] = leaf;
     rank[qi] = (int32_t)atomicAdd((unsigned long long *)&leaf_cnt[leaf], 1ull);
 }
@@ -111,134 +39,40 @@ __global__ void k_fill_i64(int64_t *p, int64_t n, int64_t v) {
 }
 
 template <typename T, int D, bool PER, int KMAX>
-__device__ __forceinline__ void knn_scan_leaf(const TreeView<T> &tv, int32_t leaf, const T (&q)[D], T (&bd)[KMAX],
-                                              int32_t (&bs)[KMAX], int64_t &tested) {
-    const Range r = tv.range[leaf];
-    for (int64_t p = r.start; p < r.start + r.count; p++) {
-        T x[D];
-        load_pt<T, D>(tv.pts, p, x);
-        const T d = dist2<T, D, PER>(x, q, tv.L);
-        if (d < bd[KMAX - 1]) list_insert<T, KMAX>(bd, bs, d, (int32_t)p);
-    }
-    tested += r.count;
-}
-
-template <typename T, int D, bool PER, int KMAX>
 __global__ __launch_bounds__(QT) void k_knn(TreeView<T> tv, const T *queries, int64_t nq, int k, int sorted,
                                             const float *seed, const int32_t *order, const int32_t *leaf_of,
                                             T *dout, int64_t *iout, int64_t *stats) {
     const int64_t t = (int64_t)blockIdx.x * QT + threadIdx.x;
-    int64_t visited = 0, tested = 0, reruns = 0;
+    Counters c;
     if (t < nq) {
-    const int64_t qi = order ? (int64_t)order[t] : t;
-    T q[D];
+        const int64_t qi = order ? (int64_t)order[t] : t;
+        T q[D];
 #pragma unroll
-    for (int d = 0; d < D; d++) q[d] = queries[qi * D + d];
-    T bd[KMAX];
-    int32_t bs[KMAX];
-    // the leaf the query falls in: nearer child at every level
-    int32_t leaf = leaf_of ? leaf_of[qi] : find_leaf<T, D>(tv, q);
-    T init = T(FLT_MAX);
-    if (seed) {
-        const T sr = (T)seed[qi];
-        const T s2 = sr * sr;
-        if (s2 < init) init = s2;
-    }
-    for (int pass = 0; pass < 2; pass++) {
-#pragma unroll
-        for (int j = 0; j < KMAX; j++) {
-            bd[j] = j < KMAX - k ? T(-INFINITY) : init;
-            bs[j] = -1;
+        for (int d = 0; d < D; d++) q[d] = queries[qi * D + d];
+        const int32_t leaf = leaf_of ? leaf_of[qi] : find_leaf<T, D>(tv, q, 0);
+        T seed2 = T(FLT_MAX);
+        if (seed) {
+            const T sr = (T)seed[qi];
+            seed2 = sr * sr;
         }
-        knn_scan_leaf<T, D, PER, KMAX>(tv, leaf, q, bd, bs, tested);
-        visited++;
-        int32_t from = leaf, cur = tv.nodes[leaf].parent;
-        while (cur >= 0) {
-            const NodeRec<T> nr = tv.nodes[cur];
-            const int32_t nearc = nr.child + (q[nr.axis] < nr.split ? 0 : 1);
-            bool moved = false;
-            if (from == nearc) {
-                const int32_t farc = nearc == nr.child ? nr.child + 1 : nr.child;
-                if (node_mindist<T, D>(tv, farc, q, PER) < bd[KMAX - 1]) {
-                    int32_t n2 = farc;
-                    for (;;) {
-                        visited++;
-                        const NodeRec<T> r2 = tv.nodes[n2];
-                        if (r2.axis < 0) {
-                            knn_scan_leaf<T, D, PER, KMAX>(tv, n2, q, bd, bs, tested);
-                            from = n2;
-                            cur = r2.parent;
-                            break;
-                        }
-                        const int32_t nc2 = r2.child + (q[r2.axis] < r2.split ? 0 : 1);
-                        if (node_mindist<T, D>(tv, nc2, q, PER) < bd[KMAX - 1]) {
-                            n2 = nc2;
-                            continue;
-                        }
-                        from = nc2;  // nearer child pruned: continue as if back from it
-                        cur = n2;
-                        break;
-                    }
-                    moved = true;
-                }
-            }
-            if (!moved) {
-                from = cur;
-                cur = nr.parent;
-            }
-        }
-        if (!seed || bs[KMAX - 1] >= 0 || init == T(FLT_MAX)) break;
-        init = T(FLT_MAX);  // fewer than k points inside the seed radius
-        reruns++;
-    }
-    // write (distance, original index) in ascending order
-    T *drow = dout + qi * k;
-    int64_t *irow = iout + qi * k;
-    bool fix = false;
-    T pd = T(-1);
-    int64_t pi = 0;
-#pragma unroll
-    for (int j = 0; j < KMAX; j++) {
-        if (j >= KMAX - k) {
-            const int32_t s = bs[j];
-            const int64_t oi = s >= 0 ? load_idx(tv.idx, tv.idx64, s) : (int64_t)CKD_EMPTY_SLOT;
-            const T dv = sqrt_rn<T>(bd[j]);
-            fix |= (dv == pd && oi < pi);
-            pd = dv;
-            pi = oi;
-            drow[j - (KMAX - k)] = dv;
-            irow[j - (KMAX - k)] = oi;
-        }
-    }
-    if (sorted && fix) {  // order equal distances by index (insertion sort, rare)
-        for (int a = 1; a < k; a++) {
-            const T dv = drow[a];
-            const int64_t iv = irow[a];
-            int b = a - 1;
-            while (b >= 0 && drow[b] == dv && irow[b] > iv) {
-                drow[b + 1] = drow[b];
-                irow[b + 1] = irow[b];
-                b--;
-            }
-            drow[b + 1] = dv;
-            irow[b + 1] = iv;
-        }
-    }
+        knn_one<T, D, PER, KMAX>(tv, q, leaf, k, sorted, seed2, dout + qi * k, iout + qi * k, 0, c);
     }
     if (stats) {  // one atomic per wave and counter
         int64_t nqw = t < nq ? 1 : 0;
 #pragma unroll
         for (int m = WAVE / 2; m >= 1; m >>= 1) {
-            visited += __shfl_xor(visited, m, WAVE);
-            tested += __shfl_xor(tested, m, WAVE);
-            reruns += __shfl_xor(reruns, m, WAVE);
+            c.visited += __shfl_xor(c.visited, m, WAVE);
+            c.tested += __shfl_xor(c.tested, m, WAVE);
+            c.reruns += __shfl_xor(c.reruns, m, WAVE);
+            c.inserted += __shfl_xor(c.inserted, m, WAVE);
             nqw += __shfl_xor(nqw, m, WAVE);
         }
         if ((threadIdx.x & (WAVE - 1)) == 0 && nqw > 0) {
-            atomicAdd((unsigned long long *)&stats[0], (unsigned long long)visited);
-            atomicAdd((unsigned long long *)&stats[1], (unsigned long long)tested);
-            atomicAdd((unsigned long long *)&stats[2], (unsigned long long)reruns);
+            atomicAdd((unsigned long long *)&stats[0], (unsigned long long)c.visited);
+            atomicAdd((unsigned long long *)&stats[1], (unsigned long long)c.tested);
+            atomicAdd((unsigned long long *)&stats[2], (unsigned long long)c.reruns);
             atomicAdd((unsigned long long *)&stats[3], (unsigned long long)nqw);
+            atomicAdd((unsigned long long *)&stats[4], (unsigned long long)c.inserted);
         }
     }
 }
@@ -289,12 +123,7 @@ __global__ __launch_bounds__(QT) void k_knn_heap(TreeView<T> tv, const T *querie
             }
         }
     };
-    int32_t leaf = 0;
-    for (;;) {
-        const NodeRec<T> nr = tv.nodes[leaf];
-        if (nr.axis < 0) break;
-        leaf = nr.child + (q[nr.axis] < nr.split ? 0 : 1);
-    }
+    const int32_t leaf = find_leaf<T, D>(tv, q, 0);
     scan(leaf);
     int32_t from = leaf, cur = tv.nodes[leaf].parent;
     while (cur >= 0) {
@@ -303,7 +132,7 @@ __global__ __launch_bounds__(QT) void k_knn_heap(TreeView<T> tv, const T *querie
         bool moved = false;
         if (from == nearc) {
             const int32_t farc = nearc == nr.child ? nr.child + 1 : nr.child;
-            if (node_mindist<T, D>(tv, farc, q, PER) < hd[0]) {
+            if (node_mindist<T, D, PER>(tv, farc, q) < hd[0]) {
                 int32_t n2 = farc;
                 for (;;) {
                     const NodeRec<T> r2 = tv.nodes[n2];
@@ -314,7 +143,7 @@ __global__ __launch_bounds__(QT) void k_knn_heap(TreeView<T> tv, const T *querie
                         break;
                     }
                     const int32_t nc2 = r2.child + (q[r2.axis] < r2.split ? 0 : 1);
-                    if (node_mindist<T, D>(tv, nc2, q, PER) < hd[0]) {
+                    if (node_mindist<T, D, PER>(tv, nc2, q) < hd[0]) {
                         n2 = nc2;
                         continue;
                     }
@@ -336,14 +165,7 @@ __global__ __launch_bounds__(QT) void k_knn_heap(TreeView<T> tv, const T *querie
     }
 }
 
-// ---------------------------------------------------------------- ball
-// Depth-first, left child first (ball_search_recursive_bbox,
-// cosmokdtree.f90:1076-1139): a cell entirely farther than r is skipped, a
-// cell entirely within r contributes its whole point range
-// (ball_reduction_recursive, :1332-1366), leaves test d^2 <= r^2 inclusively
-// (:1126).  The whole-cell bound uses the same rounded per-axis differences
-// as the point distance, so it includes exactly the points the per-point
-// test would.
+// ---------------------------------------------------------------- ball / box
 template <typename T, int D, bool PER, bool FILL>
 __global__ __launch_bounds__(QT) void k_ball(TreeView<T> tv, const T *queries, int64_t nq, T r2, int64_t *counts,
                                              const int64_t *offsets, T *dout, int64_t *iout) {
@@ -352,74 +174,11 @@ __global__ __launch_bounds__(QT) void k_ball(TreeView<T> tv, const T *queries, i
     T q[D];
 #pragma unroll
     for (int d = 0; d < D; d++) q[d] = queries[qi * D + d];
-    int64_t cnt = 0;
-    int64_t w = FILL ? offsets[qi] : 0;
-    auto emit = [&](int64_t p, T d) {
-        if (FILL) {
-            dout[w] = sqrt_rn<T>(d);
-            iout[w] = load_idx(tv.idx, tv.idx64, p);
-            w++;
-        }
-    };
-    int32_t cur = 0, from = -1;  // from == -1: arriving from above
-    while (cur >= 0) {
-        const NodeRec<T> nr = tv.nodes[cur];
-        if (from == -1) {  // first visit of cur
-            T b[2 * D];
-            load_box<T, D>(tv.bbox, cur, b);
-            const T mn = mindist2<T, D, PER>(b, q, tv.L);
-            bool descend = false;
-            if (!(mn > r2)) {
-                if (maxdist2<T, D>(b, q) <= r2) {
-                    const Range r = tv.range[cur];
-                    if (FILL)
-                        for (int64_t p = r.start; p < r.start + r.count; p++) {
-                            T x[D];
-                            load_pt<T, D>(tv.pts, p, x);
-                            emit(p, dist2<T, D, PER>(x, q, tv.L));
-                        }
-                    cnt += r.count;
-                } else if (nr.axis < 0) {
-                    const Range r = tv.range[cur];
-                    for (int64_t p = r.start; p < r.start + r.count; p++) {
-                        T x[D];
-                        load_pt<T, D>(tv.pts, p, x);
-                        const T d = dist2<T, D, PER>(x, q, tv.L);
-                        if (d <= r2) {
-                            emit(p, d);
-                            cnt++;
-                        }
-                    }
-                } else {
-                    descend = true;
-                }
-            }
-            if (descend) {
-                cur = nr.child;  // left first
-                continue;
-            }
-            from = cur;
-            cur = nr.parent;
-            continue;
-        }
-        // back from child `from`
-        if (from == nr.child) {
-            cur = nr.child + 1;
-            from = -1;
-            continue;
-        }
-        from = cur;
-        cur = nr.parent;
-    }
+    const int64_t w = FILL ? offsets[qi] : 0;
+    const int64_t cnt = ball_one<T, D, PER, FILL>(tv, q, 0, r2, FILL ? dout + w : nullptr, FILL ? iout + w : nullptr, 0);
     if (!FILL) counts[qi] = cnt;
 }
 
-// ---------------------------------------------------------------- box
-// box_search_recursive (cosmokdtree.f90:1258-1328): visit the left child iff
-// box_lo[axis] < split and the right child iff box_hi[axis] > split; leaves
-// test box_lo <= x <= box_hi on every axis.  A cell strictly inside the box
-// on every axis is taken whole: every split below it passes both visit tests
-// and every point passes the leaf test.
 template <typename T, int D, bool FILL>
 __global__ __launch_bounds__(QT) void k_box(TreeView<T> tv, const T *boxes, int64_t nb, int64_t *counts,
                                             const int64_t *offsets, int64_t *iout) {
@@ -428,57 +187,7 @@ __global__ __launch_bounds__(QT) void k_box(TreeView<T> tv, const T *boxes, int6
     T bx[2 * D];
 #pragma unroll
     for (int d = 0; d < 2 * D; d++) bx[d] = boxes[bi * 2 * D + d];
-    int64_t cnt = 0;
-    int64_t w = FILL ? offsets[bi] : 0;
-    int32_t cur = 0, from = -1;
-    while (cur >= 0) {
-        const NodeRec<T> nr = tv.nodes[cur];
-        if (from == -1) {
-            T b[2 * D];
-            load_box<T, D>(tv.bbox, cur, b);
-            bool inside = true;
-#pragma unroll
-            for (int d = 0; d < D; d++) inside = inside && bx[2 * d] < b[d] && b[D + d] < bx[2 * d + 1];
-            if (inside) {
-                const Range r = tv.range[cur];
-                if (FILL)
-                    for (int64_t p = r.start; p < r.start + r.count; p++) iout[w++] = load_idx(tv.idx, tv.idx64, p);
-                cnt += r.count;
-            } else if (nr.axis < 0) {
-                const Range r = tv.range[cur];
-                for (int64_t p = r.start; p < r.start + r.count; p++) {
-                    T x[D];
-                    load_pt<T, D>(tv.pts, p, x);
-                    bool in = true;
-#pragma unroll
-                    for (int d = 0; d < D; d++) in = in && !(x[d] < bx[2 * d] || x[d] > bx[2 * d + 1]);
-                    if (in) {
-                        if (FILL) iout[w++] = load_idx(tv.idx, tv.idx64, p);
-                        cnt++;
-                    }
-                }
-            } else {
-                if (bx[2 * nr.axis] < nr.split) {
-                    cur = nr.child;
-                    continue;
-                }
-                if (bx[2 * nr.axis + 1] > nr.split) {
-                    cur = nr.child + 1;
-                    continue;
-                }
-            }
-            from = cur;
-            cur = nr.parent;
-            continue;
-        }
-        if (from == nr.child && bx[2 * nr.axis + 1] > nr.split) {
-            cur = nr.child + 1;
-            from = -1;
-            continue;
-        }
-        from = cur;
-        cur = nr.parent;
-    }
+    const int64_t cnt = box_one<T, D, FILL>(tv, bx, 0, FILL ? iout + offsets[bi] : nullptr, 0);
     if (!FILL) counts[bi] = cnt;
 }
 

@@ -1,0 +1,318 @@
+// ckd_traverse.h -- per-query traversals shared by the single-tree kernels
+// (ckd_query.hip) and the batched persistent kernel (ckd_batch.hip).
+//
+// One thread per query.  Traversals are stack-free: a node's parent pointer
+// replaces the recursion stack of the reference, so any tree depth
+// (sliding-midpoint trees can be very deep on duplicates) is handled
+// without a stack bound.
+//
+// kNN keeps the k best candidates in registers as an ascending list
+// (distance, leaf row) of static length KMAX >= k: the k live entries are
+// the top k slots, the slots below are -inf and inert, so the pruning bound
+// is always slot KMAX-1.  A candidate is accepted only when strictly closer
+// than that bound, as in cosmokdtree.f90:658,808; the list starts filled
+// with FLT_MAX (HUGE(0.0), cosmokdtree.f90:517) or with the squared seed
+// radius.  The visit order is depth-first, nearer child first (the order
+// of knn_search_recursive_hyperp, cosmokdtree.f90:820-846), with every
+// subtree pruned by its cell's minimum distance (bbox_mindist_kd,
+// cosmokdtree.f90:1525-1546; minimum image when periodic).  Pruning never
+// drops a candidate that could be accepted, so the accepted set equals the
+// reference's up to the choice among points tied at the k-th distance.
+#pragma once
+#include "ckd_internal.h"
+
+namespace ckd {
+
+template <typename T, int D>
+__device__ __forceinline__ void load_pt(const T *pts, int64_t p, T (&x)[D]) {
+    constexpr int PS = pstride(D);
+    const T *src = pts + p * PS;
+    if constexpr (sizeof(T) == 4 && PS == 4) {
+        float4 v = *(const float4 *)src;
+        x[0] = v.x;
+        if (D > 1) x[1] = v.y;
+        if (D > 2) x[2] = v.z;
+        if (D > 3) x[3] = v.w;
+    } else {
+#pragma unroll
+        for (int d = 0; d < D; d++) x[d] = src[d];
+    }
+}
+
+template <typename T, int D>
+__device__ __forceinline__ void load_box(const T *bbox, int32_t node, T (&b)[2 * D]) {
+    const T *src = bbox + (int64_t)node * 2 * D;
+#pragma unroll
+    for (int d = 0; d < 2 * D; d++) b[d] = src[d];
+}
+
+template <typename T, int D, bool PER>
+__device__ __forceinline__ T node_mindist(const TreeView<T> &tv, int32_t node, const T (&q)[D]) {
+    T b[2 * D];
+    load_box<T, D>(tv.bbox, node, b);
+    return mindist2<T, D, PER>(b, q, tv.L);
+}
+
+template <typename T, int KMAX>
+__device__ __forceinline__ void list_insert(T (&bd)[KMAX], int32_t (&bs)[KMAX], T d, int32_t s) {
+#pragma unroll
+    for (int j = KMAX - 1; j > 0; --j) {
+        const bool sh = d < bd[j - 1];
+        const bool here = !sh && d < bd[j];
+        bd[j] = sh ? bd[j - 1] : (here ? d : bd[j]);
+        bs[j] = sh ? bs[j - 1] : (here ? s : bs[j]);
+    }
+    const bool here = d < bd[0];
+    bd[0] = here ? d : bd[0];
+    bs[0] = here ? s : bs[0];
+}
+
+// The leaf a point falls in below `root`: the nearer child at every level
+// (the first descent of knn_search_recursive_hyperp, cosmokdtree.f90:820-836).
+template <typename T, int D>
+__device__ __forceinline__ int32_t find_leaf(const TreeView<T> &tv, const T (&q)[D], int32_t root) {
+    int32_t node = root;
+    for (;;) {
+        const NodeRec<T> nr = tv.nodes[node];
+        if (nr.axis < 0) return node;
+        node = nr.child + (q[nr.axis] < nr.split ? 0 : 1);
+    }
+}
+
+struct Counters {
+    int64_t visited = 0, tested = 0, reruns = 0, inserted = 0;
+};
+
+template <typename T, int D, bool PER, int KMAX>
+__device__ __forceinline__ void knn_scan_leaf(const TreeView<T> &tv, int32_t leaf, const T (&q)[D], T (&bd)[KMAX],
+                                              int32_t (&bs)[KMAX], Counters &c) {
+    const Range r = tv.range[leaf];
+    for (int64_t p = r.start; p < r.start + r.count; p++) {
+        T x[D];
+        load_pt<T, D>(tv.pts, p, x);
+        const T d = dist2<T, D, PER>(x, q, tv.L);
+        if (d < bd[KMAX - 1]) {
+            list_insert<T, KMAX>(bd, bs, d, (int32_t)p);
+            c.inserted++;
+        }
+    }
+    c.tested += r.count;
+}
+
+// k nearest neighbours of q, starting at its leaf.  Writes k (distance,
+// index - idx_base) pairs ascending to drow/irow; with `sorted`, equal
+// distances are put in index order.  seed2 < FLT_MAX seeds the bound; a
+// search that finds fewer than k points inside it runs again unseeded.
+template <typename T, int D, bool PER, int KMAX>
+__device__ __forceinline__ void knn_one(const TreeView<T> &tv, const T (&q)[D], int32_t leaf, int k, int sorted,
+                                        T seed2, T *drow, int64_t *irow, int64_t idx_base, Counters &c) {
+    T bd[KMAX];
+    int32_t bs[KMAX];
+    T init = seed2 < T(FLT_MAX) ? seed2 : T(FLT_MAX);
+    for (int pass = 0; pass < 2; pass++) {
+#pragma unroll
+        for (int j = 0; j < KMAX; j++) {
+            bd[j] = j < KMAX - k ? T(-INFINITY) : init;
+            bs[j] = -1;
+        }
+        knn_scan_leaf<T, D, PER, KMAX>(tv, leaf, q, bd, bs, c);
+        c.visited++;
+        int32_t from = leaf, cur = tv.nodes[leaf].parent;
+        while (cur >= 0) {
+            const NodeRec<T> nr = tv.nodes[cur];
+            const int32_t nearc = nr.child + (q[nr.axis] < nr.split ? 0 : 1);
+            bool moved = false;
+            if (from == nearc) {
+                const int32_t farc = nearc == nr.child ? nr.child + 1 : nr.child;
+                if (node_mindist<T, D, PER>(tv, farc, q) < bd[KMAX - 1]) {
+                    int32_t n2 = farc;
+                    for (;;) {
+                        c.visited++;
+                        const NodeRec<T> r2 = tv.nodes[n2];
+                        if (r2.axis < 0) {
+                            knn_scan_leaf<T, D, PER, KMAX>(tv, n2, q, bd, bs, c);
+                            from = n2;
+                            cur = r2.parent;
+                            break;
+                        }
+                        const int32_t nc2 = r2.child + (q[r2.axis] < r2.split ? 0 : 1);
+                        if (node_mindist<T, D, PER>(tv, nc2, q) < bd[KMAX - 1]) {
+                            n2 = nc2;
+                            continue;
+                        }
+                        from = nc2;  // nearer child pruned: continue as if back from it
+                        cur = n2;
+                        break;
+                    }
+                    moved = true;
+                }
+            }
+            if (!moved) {
+                from = cur;
+                cur = nr.parent;
+            }
+        }
+        if (bs[KMAX - 1] >= 0 || init == T(FLT_MAX)) break;
+        init = T(FLT_MAX);  // fewer than k points inside the seed radius
+        c.reruns++;
+    }
+    bool fix = false;
+    T pd = T(-1);
+    int64_t pi = 0;
+#pragma unroll
+    for (int j = 0; j < KMAX; j++) {
+        if (j >= KMAX - k) {
+            const int32_t s = bs[j];
+            const int64_t oi = s >= 0 ? load_idx(tv.idx, tv.idx64, s) - idx_base : (int64_t)CKD_EMPTY_SLOT;
+            const T dv = sqrt_rn<T>(bd[j]);
+            fix |= (dv == pd && oi < pi);
+            pd = dv;
+            pi = oi;
+            drow[j - (KMAX - k)] = dv;
+            irow[j - (KMAX - k)] = oi;
+        }
+    }
+    if (sorted && fix) {  // order equal distances by index (insertion sort, rare)
+        for (int a = 1; a < k; a++) {
+            const T dv = drow[a];
+            const int64_t iv = irow[a];
+            int b = a - 1;
+            while (b >= 0 && drow[b] == dv && irow[b] > iv) {
+                drow[b + 1] = drow[b];
+                irow[b + 1] = irow[b];
+                b--;
+            }
+            drow[b + 1] = dv;
+            irow[b + 1] = iv;
+        }
+    }
+}
+
+// Ball search below `root`, depth-first, left child first
+// (ball_search_recursive_bbox, cosmokdtree.f90:1076-1139): a cell entirely
+// farther than r is skipped, a cell entirely within r contributes its whole
+// point range (ball_reduction_recursive, :1332-1366), leaves test
+// d^2 <= r^2 inclusively (:1126).  The whole-cell bound uses the same
+// rounded per-axis differences as the point distance, so it includes
+// exactly the points the per-point test would.  Returns the count; with
+// FILL writes (sqrt d^2, index - idx_base) from dout/iout on.
+template <typename T, int D, bool PER, bool FILL>
+__device__ __forceinline__ int64_t ball_one(const TreeView<T> &tv, const T (&q)[D], int32_t root, T r2, T *dout,
+                                            int64_t *iout, int64_t idx_base) {
+    int64_t cnt = 0, w = 0;
+    auto emit = [&](int64_t p, T d) {
+        if (FILL) {
+            dout[w] = sqrt_rn<T>(d);
+            iout[w] = load_idx(tv.idx, tv.idx64, p) - idx_base;
+            w++;
+        }
+    };
+    int32_t cur = root, from = -1;  // from == -1: arriving from above
+    while (true) {
+        const NodeRec<T> nr = tv.nodes[cur];
+        if (from == -1) {  // first visit of cur
+            T b[2 * D];
+            load_box<T, D>(tv.bbox, cur, b);
+            const T mn = mindist2<T, D, PER>(b, q, tv.L);
+            bool descend = false;
+            if (!(mn > r2)) {
+                if (maxdist2<T, D>(b, q) <= r2) {
+                    const Range r = tv.range[cur];
+                    if (FILL)
+                        for (int64_t p = r.start; p < r.start + r.count; p++) {
+                            T x[D];
+                            load_pt<T, D>(tv.pts, p, x);
+                            emit(p, dist2<T, D, PER>(x, q, tv.L));
+                        }
+                    cnt += r.count;
+                } else if (nr.axis < 0) {
+                    const Range r = tv.range[cur];
+                    for (int64_t p = r.start; p < r.start + r.count; p++) {
+                        T x[D];
+                        load_pt<T, D>(tv.pts, p, x);
+                        const T d = dist2<T, D, PER>(x, q, tv.L);
+                        if (d <= r2) {
+                            emit(p, d);
+                            cnt++;
+                        }
+                    }
+                } else {
+                    descend = true;
+                }
+            }
+            if (descend) {
+                cur = nr.child;  // left first
+                continue;
+            }
+        } else if (from == nr.child) {  // back from the left child
+            cur = nr.child + 1;
+            from = -1;
+            continue;
+        }
+        if (cur == root) break;
+        from = cur;
+        cur = nr.parent;
+    }
+    return cnt;
+}
+
+// Box search below `root` (box_search_recursive, cosmokdtree.f90:1258-1328):
+// visit the left child iff box_lo[axis] < split and the right child iff
+// box_hi[axis] > split; leaves test box_lo <= x <= box_hi on every axis.  A
+// cell strictly inside the box on every axis is taken whole: every split
+// below it passes both visit tests and every point passes the leaf test.
+template <typename T, int D, bool FILL>
+__device__ __forceinline__ int64_t box_one(const TreeView<T> &tv, const T (&bx)[2 * D], int32_t root, int64_t *iout,
+                                           int64_t idx_base) {
+    int64_t cnt = 0, w = 0;
+    int32_t cur = root, from = -1;
+    while (true) {
+        const NodeRec<T> nr = tv.nodes[cur];
+        if (from == -1) {
+            T b[2 * D];
+            load_box<T, D>(tv.bbox, cur, b);
+            bool inside = true;
+#pragma unroll
+            for (int d = 0; d < D; d++) inside = inside && bx[2 * d] < b[d] && b[D + d] < bx[2 * d + 1];
+            if (inside) {
+                const Range r = tv.range[cur];
+                if (FILL)
+                    for (int64_t p = r.start; p < r.start + r.count; p++)
+                        iout[w++] = load_idx(tv.idx, tv.idx64, p) - idx_base;
+                cnt += r.count;
+            } else if (nr.axis < 0) {
+                const Range r = tv.range[cur];
+                for (int64_t p = r.start; p < r.start + r.count; p++) {
+                    T x[D];
+                    load_pt<T, D>(tv.pts, p, x);
+                    bool in = true;
+#pragma unroll
+                    for (int d = 0; d < D; d++) in = in && !(x[d] < bx[2 * d] || x[d] > bx[2 * d + 1]);
+                    if (in) {
+                        if (FILL) iout[w++] = load_idx(tv.idx, tv.idx64, p) - idx_base;
+                        cnt++;
+                    }
+                }
+            } else {
+                if (bx[2 * nr.axis] < nr.split) {
+                    cur = nr.child;
+                    continue;
+                }
+                if (bx[2 * nr.axis + 1] > nr.split) {
+                    cur = nr.child + 1;
+                    continue;
+                }
+            }
+        } else if (from == nr.child && bx[2 * nr.axis + 1] > nr.split) {
+            cur = nr.child + 1;
+            from = -1;
+            continue;
+        }
+        if (cur == root) break;
+        from = cur;
+        cur = nr.parent;
+    }
+    return cnt;
+}
+
+}  // namespace ckd

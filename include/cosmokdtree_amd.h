@@ -60,6 +60,7 @@ typedef struct {
     int64_t nleaves;
     double boxsize[CKD_MAX_DIM];
     double build_ms;    /* device time of the last build, milliseconds */
+    int64_t nsets;      /* independent point sets (1 unless built by ckd_build_forest) */
 } ckd_tree_info;
 
 /*
@@ -73,6 +74,19 @@ typedef struct {
  */
 int ckd_build(const void *points, int64_t n, int32_t ndim, int32_t prec, int32_t leaf, const double *boxsize,
               int32_t idx_bits, void *stream, ckd_tree **out);
+
+/*
+ * Build one tree per point set, all sets in one level-synchronous pass (a
+ * forest).  Set r holds rows [set_offsets[r], set_offsets[r+1]) of `points`
+ * (set_offsets [host, nsets+1], starting at 0, strictly increasing); its
+ * root is node r and its trees' results use indices relative to the set.
+ * All sets share `boxsize` (NULL = non-periodic).  Each set's tree is the
+ * tree ckd_build would build from that set alone.  No reference counterpart
+ * (the reference builds one tree per call, cosmokdtree.f90:124-238); it
+ * serves the batched query engine (ckd_batch_*).
+ */
+int ckd_build_forest(const void *points, const int64_t *set_offsets, int64_t nsets, int32_t ndim, int32_t prec,
+                     int32_t leaf, const double *boxsize, int32_t idx_bits, void *stream, ckd_tree **out);
 
 /* Release a tree.  Replaces pydeallocate_tree (cosmokdtree2py.f90:80-97). */
 int ckd_free(ckd_tree *tree);
@@ -90,8 +104,9 @@ int ckd_get_info(const ckd_tree *tree, ckd_tree_info *info);
  * seed_r [dev, nq floats] or NULL: a per-query search radius proposal; a
  * query whose seeded search finds fewer than k points is searched again
  * without it, so results never depend on the seed.
- * stats [dev, 4 int64] or NULL: accumulates {nodes visited, leaf points
- * tested, queries re-run after a short seed, queries}.
+ * stats [dev, 5 int64] or NULL: accumulates {nodes visited, leaf points
+ * tested, queries re-run after a short seed, queries, candidates inserted
+ * into a result list}.
  */
 int ckd_knn(const ckd_tree *tree, const void *queries, int64_t nq, int32_t k, int32_t sorted, void *dist,
             int64_t *idx, const float *seed_r, int64_t *stats, void *stream);
@@ -143,6 +158,40 @@ int ckd_box_fill(const ckd_tree *tree, const void *boxes, int64_t nb, const int6
  */
 int ckd_tree_dump(const ckd_tree *tree, int32_t *axis, void *split, void *lo, void *hi, int32_t *child,
                   int32_t *parent, int64_t *start, int64_t *count, void *points, int64_t *idx);
+
+/*
+ * Batched queries over a forest (ckd_build_forest), drained by persistent
+ * kernels from a device work queue.  No reference counterpart (the
+ * reference answers one tree per call); each set's results equal the
+ * single-tree calls on that set's tree.
+ *
+ * ckd_batch_plan: set r has queries qoff[r]..qoff[r+1] [host, nsets+1] of the
+ * query array, k[r] neighbours (0..64, 0 = no kNN) and ball radius
+ * radius[r] (radius NULL or radius[r] <= 0 = no ball search).  kNN rows of
+ * set r start at element sum_{s<r} nq_s*k_s of the kNN outputs
+ * (*knn_total = total elements).  threads = queries per work item and
+ * workgroup (64, 128 or 256); blocks = persistent grid size (0 = 2048
+ * threads per CU).  Ball output holds at most ball_capacity entries.
+ *
+ * ckd_batch_run only enqueues kernels (no allocation, copy or wait), so it
+ * can be captured in a hipGraph.  Outputs [dev]: knn_dist/knn_idx
+ * (knn_total), ball_off (nq+1, CSR over all queries), ball_dist/ball_idx
+ * (ball_capacity).  Indices are relative to each set.  Ball rows are
+ * unsorted; kNN rows follow `sorted`.
+ *
+ * ckd_batch_status: waits for `stream`, then reports the ball total and
+ * whether it exceeded ball_capacity (rows past the capacity are not
+ * written).
+ */
+typedef struct ckd_batch ckd_batch;
+int ckd_batch_plan(const ckd_tree *forest, const int64_t *qoff, const int32_t *k, const double *radius,
+                   int32_t sorted, int32_t threads, int32_t blocks, int64_t ball_capacity, ckd_batch **out,
+                   int64_t *knn_total);
+int ckd_batch_run(ckd_batch *plan, const void *queries, void *knn_dist, int64_t *knn_idx, int64_t *ball_off,
+                  void *ball_dist, int64_t *ball_idx, void *stream);
+int ckd_batch_status(const ckd_batch *plan, const int64_t *ball_off, int64_t *ball_total, int32_t *overflow,
+                     void *stream);
+int ckd_batch_free(ckd_batch *plan);
 
 /*
  * Learned search-radius proposal (no reference counterpart: it only seeds

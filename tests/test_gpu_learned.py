@@ -111,7 +111,7 @@ def test_seeded_knn_is_exact(L_, clustered):
     rng = np.random.default_rng(3)
     for seeds in (np.zeros(nq), np.full(nq, 1e-3), np.full(nq, 1e3),
                   d0[:, 31].cpu().numpy() * rng.uniform(0.7, 1.3, nq), d0[:, 31].cpu().numpy()):
-        stats = torch.zeros(4, dtype=torch.int64, device="cuda")
+        stats = torch.zeros(5, dtype=torch.int64, device="cuda")
         d1, i1 = tree.knn(q, 32, sorted=True, seed_r=torch.from_numpy(seeds.astype(np.float32)), stats=stats)
         assert torch.equal(d0, d1)
         assert_knn_rows(pts, pts[::37], 32, d0.cpu().numpy(), i0.cpu().numpy(), d1.cpu().numpy(),
@@ -127,8 +127,8 @@ def test_trained_network_gains(L_, clustered, ckd):
     ttree.free()
     pts, tree = clustered
     q = torch.from_numpy(pts[:50000]).cuda()
-    s0 = torch.zeros(4, dtype=torch.int64, device="cuda")
-    s1 = torch.zeros(4, dtype=torch.int64, device="cuda")
+    s0 = torch.zeros(5, dtype=torch.int64, device="cuda")
+    s1 = torch.zeros(5, dtype=torch.int64, device="cuda")
     d0, i0 = tree.knn(q, 32, stats=s0)
     d1, i1 = L_.seeded_knn(tree, model, q, 32, stats=s1)
     assert torch.equal(d0, d1)
