@@ -130,7 +130,7 @@ def main():
         d, i = tree.knn(P, K, sorted=True)
     del d, i
     torch.cuda.synchronize()
-    stats = torch.zeros(5, dtype=torch.int64, device=dev)
+    stats = torch.zeros(6, dtype=torch.int64, device=dev)
     d, i = tree.knn(P[:1 << 20], K, sorted=True, stats=stats)  # visit counts on a 1M sample (untimed)
     st = stats.cpu().numpy()
     del d, i
@@ -164,7 +164,7 @@ def main():
         del tpts
         grid = learned.occupancy(tree)
         train_s = time.time() - t0
-        s1 = torch.zeros(5, dtype=torch.int64, device=dev)
+        s1 = torch.zeros(6, dtype=torch.int64, device=dev)
         learned.seeded_knn(tree, model, P[:1 << 20], K, grid=grid, stats=s1)
         s1 = s1.cpu().numpy()
         for _ in range(args.warmup):

@@ -13,7 +13,7 @@ import os
 import torch  # noqa: F401  (loads the HIP runtime first)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libcosmokdtree_amd.so")
+LIB_PATH = os.environ.get("CKD_LIB") or os.path.join(HERE, "libcosmokdtree_amd.so")
 
 CKD_OK = 0
 CKD_ERR_ARG = -1

@@ -34,8 +34,8 @@ FLAGS = [
 ]
 
 
-def _compile(src: str) -> tuple[str, str]:
-    obj = os.path.join(OBJ, os.path.basename(src).replace(".hip", ".o"))
+def _compile(src: str, obj_dir: str = OBJ) -> tuple[str, str]:
+    obj = os.path.join(obj_dir, os.path.basename(src).replace(".hip", ".o"))
     cmd = [HIPCC, *FLAGS, "-c", src, "-o", obj]
     p = subprocess.run(cmd, capture_output=True, text=True)
     if p.returncode != 0:
@@ -43,27 +43,30 @@ def _compile(src: str) -> tuple[str, str]:
     return obj, p.stderr
 
 
-def build(verbose: bool = False) -> str:
-    os.makedirs(OBJ, exist_ok=True)
+def build(verbose: bool = False, out: str | None = None, objdir: str | None = None) -> str:
+    """Compile and link; `out`/`objdir` redirect the library and objects."""
+    obj_dir = objdir or OBJ
+    lib_path = out or LIB
+    os.makedirs(obj_dir, exist_ok=True)
     srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
     t0 = time.time()
     with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
-        results = list(ex.map(_compile, srcs))
+        results = list(ex.map(lambda f: _compile(f, obj_dir), srcs))
     objs = [o for o, _ in results]
     if verbose:
         for (o, err) in results:
             errs = "\n".join(l for l in err.splitlines() if "packed-fp32-ops" not in l and l.strip())
             if errs:
                 print(errs, file=sys.stderr)
-    tmp = LIB + ".tmp"
+    tmp = lib_path + ".tmp"
     p = subprocess.run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp],
                        capture_output=True, text=True)
     if p.returncode != 0:
         raise RuntimeError(f"link failed:\n{p.stderr[-4000:]}")
-    os.replace(tmp, LIB)
+    os.replace(tmp, lib_path)
     if verbose:
-        print(f"built {LIB} in {time.time() - t0:.1f}s", file=sys.stderr)
-    return LIB
+        print(f"built {lib_path} in {time.time() - t0:.1f}s", file=sys.stderr)
+    return lib_path
 
 
 if __name__ == "__main__":

@@ -28,24 +28,30 @@ int hip_fail(hipError_t e, const char *what, const char *file, int line) {
     return e == hipErrorOutOfMemory ? CKD_ERR_MEMORY : CKD_ERR_HIP;
 }
 
-int workspace(ckd_tree *t, size_t bytes, void **p, hipStream_t s) {
-    if (t->ws_bytes < bytes) {
-        if (t->ws) {
+static int grow(void **buf, size_t *have, size_t bytes, void **p, hipStream_t s) {
+    if (*have < bytes) {
+        if (*buf) {
             CKD_HIP(hipStreamSynchronize(s));
-            (void)hipFree(t->ws);
-            t->ws = nullptr;
-            t->ws_bytes = 0;
+            (void)hipFree(*buf);
+            *buf = nullptr;
+            *have = 0;
         }
         const size_t want = bytes + bytes / 4;
-        hipError_t e = hipMalloc(&t->ws, want);
+        hipError_t e = hipMalloc(buf, want);
         if (e != hipSuccess) {
-            t->ws = nullptr;
+            *buf = nullptr;
             return hip_fail(e, "workspace", __FILE__, __LINE__);
         }
-        t->ws_bytes = want;
+        *have = want;
     }
-    *p = t->ws;
+    *p = *buf;
     return CKD_OK;
+}
+
+int workspace(ckd_tree *t, size_t bytes, void **p, hipStream_t s) { return grow(&t->ws, &t->ws_bytes, bytes, p, s); }
+
+int workspace2(ckd_tree *t, size_t bytes, void **p, hipStream_t s) {
+    return grow(&t->ws2, &t->ws2_bytes, bytes, p, s);
 }
 
 static void release(ckd_tree *t) {
@@ -54,12 +60,16 @@ static void release(ckd_tree *t) {
     if (t->ws) (void)hipFree(t->ws);
     t->ws = nullptr;
     t->ws_bytes = 0;
+    if (t->ws2) (void)hipFree(t->ws2);
+    t->ws2 = nullptr;
+    t->ws2_bytes = 0;
     if (t->nodes) (void)hipFree(t->nodes);
     if (t->bbox) (void)hipFree(t->bbox);
+    if (t->dbox) (void)hipFree(t->dbox);
     if (t->range) (void)hipFree(t->range);
     if (t->pts) (void)hipFree(t->pts);
     if (t->idx) (void)hipFree(t->idx);
-    t->nodes = t->bbox = t->pts = t->idx = nullptr;
+    t->nodes = t->bbox = t->dbox = t->pts = t->idx = nullptr;
     t->range = nullptr;
 }
 

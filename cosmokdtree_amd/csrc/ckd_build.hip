@@ -165,7 +165,7 @@ __global__ __launch_bounds__(BT) void k_chunk_stats(const Chunk *chunks, const i
 
 template <typename T, int D>
 __global__ __launch_bounds__(BT) void k_node_decide(const int32_t *act, const int64_t *chunk_base, const int64_t *nch,
-                                                    const Ext<T> *cstat, NodeRec<T> *nodes, T *bbox,
+                                                    const Ext<T> *cstat, NodeRec<T> *nodes, T *bbox, T *dbox,
                                                     Decision<T> *dec, int root_from_data) {
     __shared__ Ext<T> lds[BT / WAVE][D];
     const int64_t a = blockIdx.x;
@@ -180,6 +180,12 @@ __global__ __launch_bounds__(BT) void k_node_decide(const int32_t *act, const in
     block_reduce_ext<T, D>(e, lds);
     if (threadIdx.x != 0) return;
     T *bb = bbox + (int64_t)id * 2 * D;
+    T *db = dbox + (int64_t)id * 2 * D;
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        db[d] = e[d].mn;
+        db[D + d] = e[d].mx;
+    }
     if (root_from_data) {  // non-periodic root cell = data bounds (cosmokdtree.f90:226-229)
 #pragma unroll
         for (int d = 0; d < D; d++) {
@@ -412,6 +418,94 @@ __global__ __launch_bounds__(BT) void k_root_leaves(const T *P, const IdxT *I, i
         }
 }
 
+// Tight point boxes of the leaves, from the final leaf buckets.
+template <typename T, int D>
+__global__ void k_leaf_boxes(const NodeRec<T> *nodes, const Range *range, int64_t nnodes, const T *F, T *dbox) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nnodes || nodes[v].axis >= 0) return;
+    const Range r = range[v];
+    constexpr int PS = pstride(D);
+    T mn[D], mx[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        mn[d] = T(INFINITY);
+        mx[d] = T(-INFINITY);
+    }
+    for (int64_t p = r.start; p < r.start + r.count; p++)
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            const T x = F[p * PS + d];
+            mn[d] = x < mn[d] ? x : mn[d];
+            mx[d] = x > mx[d] ? x : mx[d];
+        }
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        dbox[v * 2 * D + d] = mn[d];
+        dbox[v * 2 * D + D + d] = mx[d];
+    }
+}
+
+// ---- depth-first renumbering of the sibling pairs
+// The level loop numbers nodes level by level.  Afterwards the pairs of
+// children are renumbered in depth-first order: the children of v keep
+// adjacent ids (pair base b(v)), the descendants of the first child follow
+// at b(v)+2, those of the second child after them.  A subtree's nodes are
+// then contiguous, and leaves sorted by id are in depth-first (spatial)
+// order.
+template <typename T>
+__global__ void k_subtree_size(const NodeRec<T> *nodes, int64_t lo, int64_t hi, int64_t *desc) {
+    const int64_t v = lo + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= hi) return;
+    const NodeRec<T> nr = nodes[v];
+    desc[v] = nr.axis < 0 ? 0 : 2 + desc[nr.child] + desc[nr.child + 1];
+}
+
+__global__ void k_root_ids(const int64_t *desc, const int64_t *root_off, int64_t nsets, int64_t *newid,
+                           int64_t *pairbase) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nsets) return;
+    newid[r] = r;
+    pairbase[r] = nsets + root_off[r];
+}
+
+template <typename T>
+__global__ void k_child_ids(const NodeRec<T> *nodes, int64_t lo, int64_t hi, const int64_t *desc, int64_t *newid,
+                            int64_t *pairbase) {
+    const int64_t v = lo + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= hi) return;
+    const NodeRec<T> nr = nodes[v];
+    if (nr.axis < 0) return;
+    const int64_t b = pairbase[v], c0 = nr.child, c1 = nr.child + 1;
+    newid[c0] = b;
+    newid[c1] = b + 1;
+    pairbase[c0] = b + 2;
+    pairbase[c1] = b + 2 + desc[c0];
+}
+
+template <typename T, int D>
+__global__ void k_permute_nodes(const NodeRec<T> *nodes, const T *bbox, const T *dbox, const Range *range,
+                                int64_t nn, const int64_t *newid, NodeRec<T> *nodes2, T *bbox2, T *dbox2,
+                                Range *range2) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nn) return;
+    const int64_t w = newid[v];
+    NodeRec<T> nr = nodes[v];
+    if (nr.axis >= 0) nr.child = (int32_t)newid[nr.child];
+    if (nr.parent >= 0) nr.parent = (int32_t)newid[nr.parent];
+    nodes2[w] = nr;
+#pragma unroll
+    for (int d = 0; d < 2 * D; d++) {
+        bbox2[w * 2 * D + d] = bbox[v * 2 * D + d];
+        dbox2[w * 2 * D + d] = dbox[v * 2 * D + d];
+    }
+    range2[w] = range[v];
+}
+
+__global__ void k_root_desc(const int64_t *desc, int64_t nsets, int64_t *out) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < nsets) out[r] = desc[r];
+}
+
 // Periodic range check (cosmokdtree.f90:176-190): counts points outside.
 template <typename T, int D>
 __global__ void k_check_range(const T *in, int64_t n, TreeView<T> tv, int *bad) {
@@ -445,6 +539,54 @@ struct DevBuf {
     template <typename X>
     X *as() const { return (X *)p; }
 };
+
+template <typename T, int D>
+int renumber_dfs(ckd_tree *t, const std::vector<int64_t> &lvl_start, hipStream_t s) {
+    const int64_t nn = t->nnodes, nsets = t->nsets;
+    const int nlev = (int)lvl_start.size() - 1;
+    DevBuf desc, newid, pairbase, rdesc, roff, scr;
+    int rc;
+    if ((rc = desc.alloc(sizeof(int64_t) * nn)) || (rc = newid.alloc(sizeof(int64_t) * nn)) ||
+        (rc = pairbase.alloc(sizeof(int64_t) * nn)) || (rc = rdesc.alloc(sizeof(int64_t) * nsets)) ||
+        (rc = roff.alloc(sizeof(int64_t) * (nsets + 1))) || (rc = scr.alloc(scan_tmp_bytes(nsets))))
+        return rc;
+    const NodeRec<T> *nodes = (const NodeRec<T> *)t->nodes;
+    for (int L = nlev - 1; L >= 0; L--) {  // subtree sizes, deepest level first
+        const int64_t lo = lvl_start[L], hi = lvl_start[L + 1];
+        if (hi > lo)
+            k_subtree_size<T><<<(unsigned)ceil_div(hi - lo, 256), 256, 0, s>>>(nodes, lo, hi, desc.as<int64_t>());
+    }
+    k_root_desc<<<(unsigned)ceil_div(nsets, 256), 256, 0, s>>>(desc.as<int64_t>(), nsets, rdesc.as<int64_t>());
+    if ((rc = scan_i64(rdesc.as<int64_t>(), roff.as<int64_t>(), nsets, true, scr.p, s))) return rc;
+    k_root_ids<<<(unsigned)ceil_div(nsets, 256), 256, 0, s>>>(desc.as<int64_t>(), roff.as<int64_t>(), nsets,
+                                                             newid.as<int64_t>(), pairbase.as<int64_t>());
+    for (int L = 0; L < nlev; L++) {  // ids of the children, top level first
+        const int64_t lo = lvl_start[L], hi = lvl_start[L + 1];
+        if (hi > lo)
+            k_child_ids<T><<<(unsigned)ceil_div(hi - lo, 256), 256, 0, s>>>(nodes, lo, hi, desc.as<int64_t>(),
+                                                                            newid.as<int64_t>(),
+                                                                            pairbase.as<int64_t>());
+    }
+    void *n2 = nullptr, *b2 = nullptr, *d2 = nullptr, *r2 = nullptr;
+    CKD_HIP(hipMalloc(&n2, sizeof(NodeRec<T>) * nn));
+    CKD_HIP(hipMalloc(&b2, sizeof(T) * 2 * D * nn));
+    CKD_HIP(hipMalloc(&d2, sizeof(T) * 2 * D * nn));
+    CKD_HIP(hipMalloc(&r2, sizeof(Range) * nn));
+    k_permute_nodes<T, D><<<(unsigned)ceil_div(nn, 256), 256, 0, s>>>(
+        nodes, (const T *)t->bbox, (const T *)t->dbox, t->range, nn, newid.as<int64_t>(), (NodeRec<T> *)n2, (T *)b2,
+        (T *)d2, (Range *)r2);
+    CKD_HIP(hipGetLastError());
+    CKD_HIP(hipStreamSynchronize(s));
+    (void)hipFree(t->nodes);
+    (void)hipFree(t->bbox);
+    (void)hipFree(t->dbox);
+    (void)hipFree(t->range);
+    t->nodes = n2;
+    t->bbox = b2;
+    t->dbox = d2;
+    t->range = (Range *)r2;
+    return CKD_OK;
+}
 
 template <typename T, int D, typename IdxT>
 int build_impl(ckd_tree *t, const T *in, hipStream_t s) {
@@ -486,21 +628,25 @@ int build_impl(ckd_tree *t, const T *in, hipStream_t s) {
     if (cap > 2 * n + 1) cap = 2 * n + 1;
     if (cap < 2 * nsets_ + 64) cap = 2 * nsets_ + 64;
     auto alloc_nodes = [&](int64_t c) -> int {
-        void *nn = nullptr, *nb = nullptr, *nr = nullptr;
+        void *nn = nullptr, *nb = nullptr, *nd = nullptr, *nr = nullptr;
         CKD_HIP(hipMalloc(&nn, sizeof(NodeRec<T>) * c));
         CKD_HIP(hipMalloc(&nb, sizeof(T) * 2 * D * c));
+        CKD_HIP(hipMalloc(&nd, sizeof(T) * 2 * D * c));
         CKD_HIP(hipMalloc(&nr, sizeof(Range) * c));
         if (t->nodes) {
             CKD_HIP(hipMemcpyAsync(nn, t->nodes, sizeof(NodeRec<T>) * t->nnodes, hipMemcpyDeviceToDevice, s));
             CKD_HIP(hipMemcpyAsync(nb, t->bbox, sizeof(T) * 2 * D * t->nnodes, hipMemcpyDeviceToDevice, s));
+            CKD_HIP(hipMemcpyAsync(nd, t->dbox, sizeof(T) * 2 * D * t->nnodes, hipMemcpyDeviceToDevice, s));
             CKD_HIP(hipMemcpyAsync(nr, t->range, sizeof(Range) * t->nnodes, hipMemcpyDeviceToDevice, s));
             CKD_HIP(hipStreamSynchronize(s));
             (void)hipFree(t->nodes);
             (void)hipFree(t->bbox);
+            (void)hipFree(t->dbox);
             (void)hipFree(t->range);
         }
         t->nodes = nn;
         t->bbox = nb;
+        t->dbox = nd;
         t->range = (Range *)nr;
         cap = c;
         return CKD_OK;
@@ -554,6 +700,8 @@ int build_impl(ckd_tree *t, const T *in, hipStream_t s) {
         t->nnodes = nsets;
         int64_t nleaves = nsets - nact;
         int level = 0;
+        std::vector<int64_t> lvl_start(1, 0);  // level L holds ids [lvl_start[L], lvl_start[L+1])
+        lvl_start.push_back(nsets);
         while (nact > 0) {
             if ((int64_t)next_id + 2 * nact > cap) {
                 int64_t nc = cap * 2;
@@ -578,7 +726,7 @@ int build_impl(ckd_tree *t, const T *in, hipStream_t s) {
             const int64_t *d_ntot = d_cb + nact;  // total chunks (device)
             k_chunk_stats<T, D><<<(unsigned)nub, BT, 0, s>>>(chunks.as<Chunk>(), d_ntot, Pc, n, cstat.as<Ext<T>>());
             k_node_decide<T, D><<<(unsigned)nact, BT, 0, s>>>(actc, d_cb, d_nch, cstat.as<Ext<T>>(),
-                                                              (NodeRec<T> *)t->nodes, (T *)t->bbox,
+                                                              (NodeRec<T> *)t->nodes, (T *)t->bbox, (T *)t->dbox,
                                                               dec.as<Decision<T>>(), level == 0 && !t->periodic);
             k_chunk_countleft<T><<<(unsigned)nub, BT, 0, s>>>(chunks.as<Chunk>(), d_ntot, nub, Pc, n,
                                                                dec.as<Decision<T>>(), cleft.as<int64_t>());
@@ -601,6 +749,7 @@ int build_impl(ckd_tree *t, const T *in, hipStream_t s) {
             const int64_t nact_next = *h_nact;
             nleaves += m - nact_next;
             next_id += (int32_t)m;
+            lvl_start.push_back(next_id);
             nact = nact_next;
             std::swap(Pc, Pn);
             std::swap(Ic, In);
@@ -612,6 +761,10 @@ int build_impl(ckd_tree *t, const T *in, hipStream_t s) {
         t->nnodes = next_id;
         t->nleaves = nleaves;
         t->depth = level;
+        k_leaf_boxes<T, D><<<(unsigned)ceil_div(t->nnodes, 256), 256, 0, s>>>(
+            (const NodeRec<T> *)t->nodes, t->range, t->nnodes, (const T *)t->pts, (T *)t->dbox);
+        CKD_HIP(hipGetLastError());
+        if ((rc = renumber_dfs<T, D>(t, lvl_start, s))) return rc;
     }
     CKD_HIP(hipEventRecord(ev1, s));
     CKD_HIP(hipEventSynchronize(ev1));

@@ -6,6 +6,8 @@
 //                   (allocated as a pair), parent[0] = -1.
 //   bbox[nnodes]    2*D values of T: the node's cell, lo[0..D) then hi[0..D)
 //                   (KDTreeNode%minbounds/maxbounds, cosmokdtree.f90:99-100).
+//   dbox[nnodes]    2*D values of T: the tight bounding box of the node's
+//                   points (inside the cell; what the traversals prune with).
 //   range[nnodes]   {start, count} int64: the node's points are rows
 //                   [start, start+count) of the leaf-ordered arrays below,
 //                   for inner nodes too (a subtree is one contiguous range).
@@ -45,6 +47,7 @@ template <typename T>
 struct TreeView {
     const NodeRec<T> *nodes;
     const T *bbox;
+    const T *dbox;
     const Range *range;
     const T *pts;
     const void *idx;
@@ -201,7 +204,8 @@ struct ckd_tree {
     double L[CKD_MAX_DIM] = {0};
     double build_ms = 0;
     void *nodes = nullptr;   // NodeRec<T>[nnodes]
-    void *bbox = nullptr;    // T[nnodes * 2 * ndim]
+    void *bbox = nullptr;    // T[nnodes * 2 * ndim] cells
+    void *dbox = nullptr;    // T[nnodes * 2 * ndim] tight point boxes
     ckd::Range *range = nullptr;
     void *pts = nullptr;     // T[n * pstride(ndim)]
     void *idx = nullptr;     // int32 / int64 [n]
@@ -209,6 +213,8 @@ struct ckd_tree {
     int64_t *set_off = nullptr;  // device [nsets + 1]: set r holds input rows [set_off[r], set_off[r+1])
     void *ws = nullptr;      // grow-only query workspace (calls on one tree must not overlap)
     size_t ws_bytes = 0;
+    void *ws2 = nullptr;     // second workspace (retry lists)
+    size_t ws2_bytes = 0;
 };
 
 namespace ckd {
@@ -217,6 +223,7 @@ TreeView<T> view(const ckd_tree *t) {
     TreeView<T> v;
     v.nodes = (const NodeRec<T> *)t->nodes;
     v.bbox = (const T *)t->bbox;
+    v.dbox = (const T *)t->dbox;
     v.range = t->range;
     v.pts = (const T *)t->pts;
     v.idx = t->idx;
@@ -228,4 +235,5 @@ int build_tree(ckd_tree *t, const void *points, hipStream_t s);
 // Device workspace of at least `bytes` owned by the tree (grown on demand;
 // growing synchronises `s` before releasing the old buffer).
 int workspace(ckd_tree *t, size_t bytes, void **p, hipStream_t s);
+int workspace2(ckd_tree *t, size_t bytes, void **p, hipStream_t s);
 }  // namespace ckd

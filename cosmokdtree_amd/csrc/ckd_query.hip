@@ -1,6 +1,8 @@
 // ckd_query.hip -- single-tree kNN, ball and box kernels (one thread per
 // query) and their host dispatch; the traversals themselves are in
 // ckd_traverse.h.
+#include <cstdlib>
+
 #include "ckd_traverse.h"
 
 namespace ckd {
@@ -31,6 +33,11 @@ __global__ void k_order(const int32_t *leaf_of, const int32_t *rank, const int64
     const int64_t qi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (qi >= nq) return;
     order[leaf_off[leaf_of[qi]] + rank[qi]] = (int32_t)qi;
+}
+
+__global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
 __global__ void k_fill_i64(int64_t *p, int64_t n, int64_t v) {
@@ -65,6 +72,7 @@ __global__ __launch_bounds__(QT) void k_knn(TreeView<T> tv, const T *queries, in
             c.tested += __shfl_xor(c.tested, m, WAVE);
             c.reruns += __shfl_xor(c.reruns, m, WAVE);
             c.inserted += __shfl_xor(c.inserted, m, WAVE);
+            c.climbed += __shfl_xor(c.climbed, m, WAVE);
             nqw += __shfl_xor(nqw, m, WAVE);
         }
         if ((threadIdx.x & (WAVE - 1)) == 0 && nqw > 0) {
@@ -73,6 +81,53 @@ __global__ __launch_bounds__(QT) void k_knn(TreeView<T> tv, const T *queries, in
             atomicAdd((unsigned long long *)&stats[2], (unsigned long long)c.reruns);
             atomicAdd((unsigned long long *)&stats[3], (unsigned long long)nqw);
             atomicAdd((unsigned long long *)&stats[4], (unsigned long long)c.inserted);
+            atomicAdd((unsigned long long *)&stats[5], (unsigned long long)c.climbed);
+        }
+    }
+}
+
+// Wave-uniform variant (knn_wave): queries in leaf order, one traversal per
+// wave.
+template <typename T, int D, bool PER, int KMAX>
+__global__ __launch_bounds__(QT, 3) void k_knn_w(TreeView<T> tv, const T *queries, int64_t nq, int k, int sorted,
+                                              const float *seed, const int32_t *order, T *dout, int64_t *iout,
+                                              int64_t *stats, int32_t *retry, int32_t *nretry) {
+    __shared__ T buf_d[CBUF * QT];
+    __shared__ int32_t buf_s[CBUF * QT];
+    const int64_t t = (int64_t)blockIdx.x * QT + threadIdx.x;
+    Counters c;
+    const bool active = t < nq;
+    const int64_t qi = active ? (order ? (int64_t)order[t] : t) : 0;
+    T q[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) q[d] = active ? queries[qi * D + d] : T(0);
+    T seed2 = T(FLT_MAX);
+    if (seed && active) {
+        const T sr = (T)seed[qi];
+        seed2 = sr * sr;
+    }
+    const bool done = knn_wave<T, D, PER, KMAX>(tv, q, active, 0, k, sorted, seed2, dout + qi * k, iout + qi * k,
+                                                0, c, buf_d, buf_s, threadIdx.x, QT);
+    if (!done) retry[atomicAdd(nretry, 1)] = (int32_t)qi;
+    if (stats) {
+        int64_t nqw = active ? 1 : 0;
+        if (!active) c = Counters();
+#pragma unroll
+        for (int m = WAVE / 2; m >= 1; m >>= 1) {
+            c.visited += __shfl_xor(c.visited, m, WAVE);
+            c.tested += __shfl_xor(c.tested, m, WAVE);
+            c.reruns += __shfl_xor(c.reruns, m, WAVE);
+            c.inserted += __shfl_xor(c.inserted, m, WAVE);
+            c.climbed += __shfl_xor(c.climbed, m, WAVE);
+            nqw += __shfl_xor(nqw, m, WAVE);
+        }
+        if ((threadIdx.x & (WAVE - 1)) == 0 && nqw > 0) {
+            atomicAdd((unsigned long long *)&stats[0], (unsigned long long)c.visited);
+            atomicAdd((unsigned long long *)&stats[1], (unsigned long long)c.tested);
+            atomicAdd((unsigned long long *)&stats[2], (unsigned long long)c.reruns);
+            atomicAdd((unsigned long long *)&stats[3], (unsigned long long)nqw);
+            atomicAdd((unsigned long long *)&stats[4], (unsigned long long)c.inserted);
+            atomicAdd((unsigned long long *)&stats[5], (unsigned long long)c.climbed);
         }
     }
 }
@@ -193,6 +248,17 @@ __global__ __launch_bounds__(QT) void k_box(TreeView<T> tv, const T *boxes, int6
 
 // ------------------------------------------------------------ dispatch
 
+// 0 = one traversal per query (k_knn), 1 = one traversal per wave
+// (k_knn_w); CKD_KNN_KERNEL=lane|wave overrides the default.
+int knn_kernel_choice() {
+    static int choice = -1;
+    if (choice < 0) {
+        const char *e = getenv("CKD_KNN_KERNEL");
+        choice = (e && e[0] == 'w') ? 1 : 0;
+    }
+    return choice;
+}
+
 template <typename T, int D, bool PER>
 int knn_launch(const ckd_tree *t, const T *q, int64_t nq, int k, int sorted, const float *seed, T *dist,
                int64_t *idx, int64_t *stats, hipStream_t s) {
@@ -219,10 +285,41 @@ int knn_launch(const ckd_tree *t, const T *q, int64_t nq, int k, int sorted, con
             order = ord;
             leaf_of = lo;
         }
-        if (k <= 8) k_knn<T, D, PER, 8><<<g, QT, 0, s>>>(tv, q, nq, k, sorted, seed, order, leaf_of, dist, idx, stats);
-        else if (k <= 16) k_knn<T, D, PER, 16><<<g, QT, 0, s>>>(tv, q, nq, k, sorted, seed, order, leaf_of, dist, idx, stats);
-        else if (k <= 32) k_knn<T, D, PER, 32><<<g, QT, 0, s>>>(tv, q, nq, k, sorted, seed, order, leaf_of, dist, idx, stats);
-        else k_knn<T, D, PER, 64><<<g, QT, 0, s>>>(tv, q, nq, k, sorted, seed, order, leaf_of, dist, idx, stats);
+        if (knn_kernel_choice() == 1) {
+            // seeded queries whose seed held fewer than k points are listed
+            // in `retry` and searched again by the per-query kernel
+            int32_t *retry = nullptr, *nretry = nullptr;
+            if (seed) {
+                void *ws2 = nullptr;
+                int rc = workspace2(const_cast<ckd_tree *>(t), sizeof(int32_t) * (size_t)(nq + 64), &ws2, s);
+                if (rc) return rc;
+                nretry = (int32_t *)ws2;
+                retry = nretry + 64;
+                k_fill_i32<<<1, 64, 0, s>>>(nretry, 1, 0);
+            }
+            if (k <= 8) k_knn_w<T, D, PER, 8><<<g, QT, 0, s>>>(tv, q, nq, k, sorted, seed, order, dist, idx, stats, retry, nretry);
+            else if (k <= 16) k_knn_w<T, D, PER, 16><<<g, QT, 0, s>>>(tv, q, nq, k, sorted, seed, order, dist, idx, stats, retry, nretry);
+            else if (k <= 32) k_knn_w<T, D, PER, 32><<<g, QT, 0, s>>>(tv, q, nq, k, sorted, seed, order, dist, idx, stats, retry, nretry);
+            else k_knn_w<T, D, PER, 64><<<g, QT, 0, s>>>(tv, q, nq, k, sorted, seed, order, dist, idx, stats, retry, nretry);
+            CKD_HIP(hipGetLastError());
+            if (seed) {
+                int32_t h = 0;
+                CKD_HIP(hipMemcpyAsync(&h, nretry, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+                CKD_HIP(hipStreamSynchronize(s));
+                if (h > 0) {
+                    const unsigned g2 = (unsigned)ceil_div(h, QT);
+                    if (k <= 8) k_knn<T, D, PER, 8><<<g2, QT, 0, s>>>(tv, q, h, k, sorted, nullptr, retry, nullptr, dist, idx, stats);
+                    else if (k <= 16) k_knn<T, D, PER, 16><<<g2, QT, 0, s>>>(tv, q, h, k, sorted, nullptr, retry, nullptr, dist, idx, stats);
+                    else if (k <= 32) k_knn<T, D, PER, 32><<<g2, QT, 0, s>>>(tv, q, h, k, sorted, nullptr, retry, nullptr, dist, idx, stats);
+                    else k_knn<T, D, PER, 64><<<g2, QT, 0, s>>>(tv, q, h, k, sorted, nullptr, retry, nullptr, dist, idx, stats);
+                }
+            }
+        } else {
+            if (k <= 8) k_knn<T, D, PER, 8><<<g, QT, 0, s>>>(tv, q, nq, k, sorted, seed, order, leaf_of, dist, idx, stats);
+            else if (k <= 16) k_knn<T, D, PER, 16><<<g, QT, 0, s>>>(tv, q, nq, k, sorted, seed, order, leaf_of, dist, idx, stats);
+            else if (k <= 32) k_knn<T, D, PER, 32><<<g, QT, 0, s>>>(tv, q, nq, k, sorted, seed, order, leaf_of, dist, idx, stats);
+            else k_knn<T, D, PER, 64><<<g, QT, 0, s>>>(tv, q, nq, k, sorted, seed, order, leaf_of, dist, idx, stats);
+        }
         CKD_HIP(hipGetLastError());
         return CKD_OK;
     }

@@ -14,8 +14,10 @@
 // with FLT_MAX (HUGE(0.0), cosmokdtree.f90:517) or with the squared seed
 // radius.  The visit order is depth-first, nearer child first (the order
 // of knn_search_recursive_hyperp, cosmokdtree.f90:820-846), with every
-// subtree pruned by its cell's minimum distance (bbox_mindist_kd,
-// cosmokdtree.f90:1525-1546; minimum image when periodic).  Pruning never
+// subtree pruned by the minimum distance to the tight box of its points
+// (the reference prunes with the cell, bbox_mindist_kd,
+// cosmokdtree.f90:1525-1546; the points lie in both, so the tighter box
+// prunes more and drops nothing; minimum image when periodic).  Pruning never
 // drops a candidate that could be accepted, so the accepted set equals the
 // reference's up to the choice among points tied at the k-th distance.
 #pragma once
@@ -49,7 +51,7 @@ __device__ __forceinline__ void load_box(const T *bbox, int32_t node, T (&b)[2 *
 template <typename T, int D, bool PER>
 __device__ __forceinline__ T node_mindist(const TreeView<T> &tv, int32_t node, const T (&q)[D]) {
     T b[2 * D];
-    load_box<T, D>(tv.bbox, node, b);
+    load_box<T, D>(tv.dbox, node, b);
     return mindist2<T, D, PER>(b, q, tv.L);
 }
 
@@ -79,8 +81,10 @@ __device__ __forceinline__ int32_t find_leaf(const TreeView<T> &tv, const T (&q)
     }
 }
 
-struct Counters {
-    int64_t visited = 0, tested = 0, reruns = 0, inserted = 0;
+constexpr int CBUF = 16;  // leaf candidates buffered per lane between list updates
+
+struct Counters {  // per-query work counts (kernel statistics)
+    int64_t visited = 0, tested = 0, reruns = 0, inserted = 0, climbed = 0;
 };
 
 template <typename T, int D, bool PER, int KMAX>
@@ -119,6 +123,7 @@ __device__ __forceinline__ void knn_one(const TreeView<T> &tv, const T (&q)[D], 
         c.visited++;
         int32_t from = leaf, cur = tv.nodes[leaf].parent;
         while (cur >= 0) {
+            c.climbed++;
             const NodeRec<T> nr = tv.nodes[cur];
             const int32_t nearc = nr.child + (q[nr.axis] < nr.split ? 0 : 1);
             bool moved = false;
@@ -188,13 +193,176 @@ __device__ __forceinline__ void knn_one(const TreeView<T> &tv, const T (&q)[D], 
     }
 }
 
+// Wave-uniform kNN: the 64 queries of a wave share one depth-first
+// traversal of the tree below `root`.  Node records, boxes and leaf points
+// are read once per wave (uniform addresses); a child is visited when any
+// lane's own bound still reaches its point box, children in near-first
+// order for the mean of the wave's queries; at a leaf every lane tests every
+// point against its own query with its own strict bound.  Every lane visits
+// a superset of the nodes its own search would need, in a fixed order, so
+// each lane's accepted set is exactly the k nearest (ties at the k-th
+// distance aside).  All lanes of the wave must call it; lanes without a
+// query pass active = false.  One pass: a seeded lane that finds fewer than
+// k points inside its seed writes nothing and returns false, so the caller
+// can search it again unseeded (seeded waves never pay for a second pass).
+template <typename T>
+__device__ __forceinline__ T wave_uniform(T v) {
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(T, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int32_t, v)));
+    } else {
+        const int64_t b = __builtin_bit_cast(int64_t, v);
+        const int32_t lo = __builtin_amdgcn_readfirstlane((int32_t)(b & 0xffffffff));
+        const int32_t hi = __builtin_amdgcn_readfirstlane((int32_t)(b >> 32));
+        return __builtin_bit_cast(T, (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
+    }
+}
+
+template <typename T, int D, bool PER, int KMAX>
+__device__ __forceinline__ bool knn_wave(const TreeView<T> &tv, const T (&q)[D], bool active, int32_t root, int k,
+                                         int sorted, T seed2, T *drow, int64_t *irow, int64_t idx_base,
+                                         Counters &c, T *buf_d, int32_t *buf_s, int tid, int nthreads) {
+    T bd[KMAX];
+    int32_t bs[KMAX];
+    // near-first reference: mean of the active queries, made wave-uniform
+    T ref[D];
+    {
+        T n = active ? T(1) : T(0);
+#pragma unroll
+        for (int m = WAVE / 2; m >= 1; m >>= 1) n += __shfl_xor(n, m, WAVE);
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            T v = active ? q[d] : T(0);
+#pragma unroll
+            for (int m = WAVE / 2; m >= 1; m >>= 1) v += __shfl_xor(v, m, WAVE);
+            ref[d] = wave_uniform<T>(n > T(0) ? v / n : T(0));
+        }
+    }
+    const T init = seed2 < T(FLT_MAX) ? seed2 : T(FLT_MAX);
+    const bool run = active;
+    {
+#pragma unroll
+        for (int j = 0; j < KMAX; j++) {
+            bd[j] = j < KMAX - k ? T(-INFINITY) : init;
+            bs[j] = -1;
+        }
+        int32_t cur = root, from = -1;
+        for (;;) {
+            const NodeRec<T> nr = tv.nodes[cur];
+            if (from == -1) {
+                c.visited++;
+                if (nr.axis < 0) {
+                    // every lane tests the leaf's points (uniform loads); the
+                    // candidates are buffered in LDS and inserted in rounds,
+                    // so the list update runs as often as the busiest lane
+                    // needs, not once per point some lane accepts
+                    // Leaves longer than CBUF points go in chunks of CBUF,
+                    // so a lane never buffers more than CBUF candidates;
+                    // candidates are inserted in point order and re-tested
+                    // against the current k-th distance, as in a plain scan.
+                    const Range r = tv.range[cur];
+                    const int32_t pb = (int32_t)r.start, pe = (int32_t)(r.start + r.count);
+#pragma unroll 1
+                    for (int32_t cb = pb; cb < pe; cb += CBUF) {
+                        const int32_t ce = cb + CBUF < pe ? cb + CBUF : pe;
+                        const T bound = bd[KMAX - 1];
+                        int cnt = 0;
+                        for (int32_t p = cb; p < ce; p++) {
+                            T x[D];
+                            load_pt<T, D>(tv.pts, p, x);
+                            const T d = dist2<T, D, PER>(x, q, tv.L);
+                            if (run && d < bound) {
+                                buf_d[cnt * nthreads + tid] = d;
+                                buf_s[cnt * nthreads + tid] = p;
+                                cnt++;
+                            }
+                        }
+                        const int jmax = __reduce_max_sync(~0ull, cnt);
+#pragma unroll 1
+                        for (int j = 0; j < jmax; j++) {
+                            if (j < cnt) {
+                                const T d = buf_d[j * nthreads + tid];
+                                if (d < bd[KMAX - 1]) {
+                                    list_insert<T, KMAX>(bd, bs, d, buf_s[j * nthreads + tid]);
+                                    c.inserted++;
+                                }
+                            }
+                        }
+                    }
+                    c.tested += pe - pb;
+                } else {
+                    const int32_t nearc = nr.child + (ref[nr.axis] < nr.split ? 0 : 1);
+                    const int32_t farc = nearc == nr.child ? nr.child + 1 : nr.child;
+                    if (__any(run && node_mindist<T, D, PER>(tv, nearc, q) < bd[KMAX - 1])) {
+                        cur = nearc;
+                        continue;
+                    }
+                    if (__any(run && node_mindist<T, D, PER>(tv, farc, q) < bd[KMAX - 1])) {
+                        cur = farc;
+                        continue;
+                    }
+                }
+            } else {
+                c.climbed++;
+                const int32_t nearc = nr.child + (ref[nr.axis] < nr.split ? 0 : 1);
+                const int32_t farc = nearc == nr.child ? nr.child + 1 : nr.child;
+                if (from == nearc && __any(run && node_mindist<T, D, PER>(tv, farc, q) < bd[KMAX - 1])) {
+                    cur = farc;
+                    from = -1;
+                    continue;
+                }
+            }
+            if (cur == root) break;
+            from = cur;
+            cur = nr.parent;
+        }
+    }
+    // a seeded search that found fewer than k points is incomplete: the
+    // caller runs it again unseeded
+    if (!active) return true;
+    if (bs[KMAX - 1] < 0 && init != T(FLT_MAX)) {
+        c.reruns++;
+        return false;
+    }
+    bool fix = false;
+    T pd = T(-1);
+    int64_t pi = 0;
+#pragma unroll
+    for (int j = 0; j < KMAX; j++) {
+        if (j >= KMAX - k) {
+            const int32_t s = bs[j];
+            const int64_t oi = s >= 0 ? load_idx(tv.idx, tv.idx64, s) - idx_base : (int64_t)CKD_EMPTY_SLOT;
+            const T dv = sqrt_rn<T>(bd[j]);
+            fix |= (dv == pd && oi < pi);
+            pd = dv;
+            pi = oi;
+            drow[j - (KMAX - k)] = dv;
+            irow[j - (KMAX - k)] = oi;
+        }
+    }
+    if (sorted && fix) {  // order equal distances by index (insertion sort, rare)
+        for (int a = 1; a < k; a++) {
+            const T dv = drow[a];
+            const int64_t iv = irow[a];
+            int b = a - 1;
+            while (b >= 0 && drow[b] == dv && irow[b] > iv) {
+                drow[b + 1] = drow[b];
+                irow[b + 1] = irow[b];
+                b--;
+            }
+            drow[b + 1] = dv;
+            irow[b + 1] = iv;
+        }
+    }
+    return true;
+}
+
 // Ball search below `root`, depth-first, left child first
-// (ball_search_recursive_bbox, cosmokdtree.f90:1076-1139): a cell entirely
-// farther than r is skipped, a cell entirely within r contributes its whole
-// point range (ball_reduction_recursive, :1332-1366), leaves test
-// d^2 <= r^2 inclusively (:1126).  The whole-cell bound uses the same
-// rounded per-axis differences as the point distance, so it includes
-// exactly the points the per-point test would.  Returns the count; with
+// (ball_search_recursive_bbox, cosmokdtree.f90:1076-1139): a subtree whose
+// point box is entirely farther than r is skipped, one entirely within r
+// contributes its whole point range (ball_reduction_recursive,
+// :1332-1366), leaves test d^2 <= r^2 inclusively (:1126).  The box bounds
+// use the same rounded per-axis differences as the point distance, so the
+// whole-range rule includes exactly the points the per-point test would.  Returns the count; with
 // FILL writes (sqrt d^2, index - idx_base) from dout/iout on.
 template <typename T, int D, bool PER, bool FILL>
 __device__ __forceinline__ int64_t ball_one(const TreeView<T> &tv, const T (&q)[D], int32_t root, T r2, T *dout,
@@ -212,7 +380,7 @@ __device__ __forceinline__ int64_t ball_one(const TreeView<T> &tv, const T (&q)[
         const NodeRec<T> nr = tv.nodes[cur];
         if (from == -1) {  // first visit of cur
             T b[2 * D];
-            load_box<T, D>(tv.bbox, cur, b);
+            load_box<T, D>(tv.dbox, cur, b);
             const T mn = mindist2<T, D, PER>(b, q, tv.L);
             bool descend = false;
             if (!(mn > r2)) {
@@ -259,8 +427,11 @@ __device__ __forceinline__ int64_t ball_one(const TreeView<T> &tv, const T (&q)[
 // Box search below `root` (box_search_recursive, cosmokdtree.f90:1258-1328):
 // visit the left child iff box_lo[axis] < split and the right child iff
 // box_hi[axis] > split; leaves test box_lo <= x <= box_hi on every axis.  A
-// cell strictly inside the box on every axis is taken whole: every split
-// below it passes both visit tests and every point passes the leaf test.
+// subtree whose point box lies strictly inside the query box is taken
+// whole: each split lies within its node's point range, so every split
+// below passes both visit tests and every point passes the leaf test.  A
+// subtree whose point box misses the query box is skipped: none of its
+// points could pass the leaf test.
 template <typename T, int D, bool FILL>
 __device__ __forceinline__ int64_t box_one(const TreeView<T> &tv, const T (&bx)[2 * D], int32_t root, int64_t *iout,
                                            int64_t idx_base) {
@@ -270,11 +441,16 @@ __device__ __forceinline__ int64_t box_one(const TreeView<T> &tv, const T (&bx)[
         const NodeRec<T> nr = tv.nodes[cur];
         if (from == -1) {
             T b[2 * D];
-            load_box<T, D>(tv.bbox, cur, b);
-            bool inside = true;
+            load_box<T, D>(tv.dbox, cur, b);
+            bool inside = true, apart = false;
 #pragma unroll
-            for (int d = 0; d < D; d++) inside = inside && bx[2 * d] < b[d] && b[D + d] < bx[2 * d + 1];
-            if (inside) {
+            for (int d = 0; d < D; d++) {
+                inside = inside && bx[2 * d] < b[d] && b[D + d] < bx[2 * d + 1];
+                apart = apart || b[D + d] < bx[2 * d] || b[d] > bx[2 * d + 1];
+            }
+            if (apart) {
+                // no point of the subtree can pass the inclusive leaf test
+            } else if (inside) {
                 const Range r = tv.range[cur];
                 if (FILL)
                     for (int64_t p = r.start; p < r.start + r.count; p++)

@@ -104,9 +104,9 @@ int ckd_get_info(const ckd_tree *tree, ckd_tree_info *info);
  * seed_r [dev, nq floats] or NULL: a per-query search radius proposal; a
  * query whose seeded search finds fewer than k points is searched again
  * without it, so results never depend on the seed.
- * stats [dev, 5 int64] or NULL: accumulates {nodes visited, leaf points
+ * stats [dev, 6 int64] or NULL: accumulates {nodes visited, leaf points
  * tested, queries re-run after a short seed, queries, candidates inserted
- * into a result list}.
+ * into a result list, parent steps climbed}.
  */
 int ckd_knn(const ckd_tree *tree, const void *queries, int64_t nq, int32_t k, int32_t sorted, void *dist,
             int64_t *idx, const float *seed_r, int64_t *stats, void *stream);
