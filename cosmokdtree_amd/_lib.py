@@ -26,8 +26,9 @@ CKD_MAX_DIM = 8
 
 EXPORTS = (
     "ckd_build", "ckd_build_forest", "ckd_free", "ckd_get_info", "ckd_knn", "ckd_ball_count", "ckd_ball_fill",
-    "ckd_box_count", "ckd_box_fill", "ckd_tree_dump", "ckd_last_error", "ckd_version",
+    "ckd_box_count", "ckd_box_fill", "ckd_tree_dump", "ckd_last_error", "ckd_version", "ckd_knn_config",
 )
+KNN_KERNELS = {"auto": -1, "lane": 0, "wave": 1, "stack": 2}
 
 
 class TreeInfo(C.Structure):
@@ -50,6 +51,7 @@ def _load():
     lib.ckd_free.argtypes = [vp]
     lib.ckd_get_info.argtypes = [vp, C.POINTER(TreeInfo)]
     lib.ckd_knn.argtypes = [vp, vp, i64, i32, i32, vp, vp, vp, vp, vp]
+    lib.ckd_knn_config.argtypes = [vp, i32, i32]
     lib.ckd_ball_count.argtypes = [vp, vp, i64, dbl, vp, C.POINTER(i64), vp]
     lib.ckd_ball_fill.argtypes = [vp, vp, i64, dbl, i32, vp, vp, vp, vp]
     lib.ckd_box_count.argtypes = [vp, vp, i64, vp, C.POINTER(i64), vp]

@@ -69,7 +69,11 @@ static void release(ckd_tree *t) {
     if (t->range) (void)hipFree(t->range);
     if (t->pts) (void)hipFree(t->pts);
     if (t->idx) (void)hipFree(t->idx);
-    t->nodes = t->bbox = t->dbox = t->pts = t->idx = nullptr;
+    if (t->wide) (void)hipFree(t->wide);
+    if (t->wroot) (void)hipFree(t->wroot);
+    t->nodes = t->bbox = t->dbox = t->pts = t->idx = t->wide = nullptr;
+    t->wroot = nullptr;
+    t->nwide = 0;
     t->range = nullptr;
 }
 
@@ -252,6 +256,25 @@ int ckd_knn(const ckd_tree *t, const void *queries, int64_t nq, int32_t k, int32
     }
     if (nq == 0) return CKD_OK;
     return query_knn(t, queries, nq, k, sorted, seed_r, dist, idx, stats, (hipStream_t)stream);
+}
+
+int ckd_knn_config(ckd_tree *t, int32_t kernel, int32_t stack_depth) {
+    g_err[0] = 0;
+    if (!t) {
+        set_error("tree must not be NULL");
+        return CKD_ERR_ARG;
+    }
+    if (kernel < CKD_KNN_AUTO || kernel > CKD_KNN_STACK) {
+        set_error("unknown kNN kernel %d", kernel);
+        return CKD_ERR_ARG;
+    }
+    if (stack_depth != 0 && stack_depth != 32 && stack_depth != 64) {
+        set_error("stack depth must be 0, 32 or 64, got %d", stack_depth);
+        return CKD_ERR_ARG;
+    }
+    t->knn_kernel = kernel;
+    t->knn_stack = stack_depth;
+    return CKD_OK;
 }
 
 int ckd_ball_count(const ckd_tree *t, const void *queries, int64_t nq, double radius, int64_t *offsets,

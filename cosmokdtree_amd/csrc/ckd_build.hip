@@ -506,6 +506,45 @@ __global__ void k_root_desc(const int64_t *desc, int64_t nsets, int64_t *out) {
     if (r < nsets) out[r] = desc[r];
 }
 
+// ---- wide records (layout in ckd_internal.h)
+template <typename T>
+__global__ void k_inner_flags(const NodeRec<T> *nodes, int64_t nn, int64_t *flag) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < nn) flag[v] = nodes[v].axis >= 0 ? 1 : 0;
+}
+
+template <typename T>
+__device__ __forceinline__ int2 child_pair(const NodeRec<T> *nodes, const Range *range, const int64_t *widx,
+                                           int64_t c) {
+    if (nodes[c].axis >= 0) return make_int2((int32_t)widx[c], -1);
+    const Range r = range[c];
+    return make_int2((int32_t)r.start, (int32_t)r.count);
+}
+
+template <typename T, int D>
+__global__ void k_fill_wide(const NodeRec<T> *nodes, const T *dbox, const Range *range, const int64_t *widx,
+                            int64_t nn, char *wide) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nn) return;
+    const NodeRec<T> nr = nodes[v];
+    if (nr.axis < 0) return;
+    char *base = wide + widx[v] * wide_bytes<T, D>();
+    T *bx = (T *)base;
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+#pragma unroll
+        for (int d = 0; d < 2 * D; d++) bx[c * 2 * D + d] = dbox[(int64_t)(nr.child + c) * 2 * D + d];
+    const int2 p0 = child_pair(nodes, range, widx, nr.child), p1 = child_pair(nodes, range, widx, nr.child + 1);
+    *(int4 *)(base + 4 * D * sizeof(T)) = make_int4(p0.x, p0.y, p1.x, p1.y);
+}
+
+template <typename T>
+__global__ void k_root_pairs(const NodeRec<T> *nodes, const Range *range, const int64_t *widx, int64_t nsets,
+                             int2 *wroot) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < nsets) wroot[r] = child_pair(nodes, range, widx, r);
+}
+
 // Periodic range check (cosmokdtree.f90:176-190): counts points outside.
 template <typename T, int D>
 __global__ void k_check_range(const T *in, int64_t n, TreeView<T> tv, int *bad) {
@@ -585,6 +624,36 @@ int renumber_dfs(ckd_tree *t, const std::vector<int64_t> &lvl_start, hipStream_t
     t->bbox = b2;
     t->dbox = d2;
     t->range = (Range *)r2;
+    return CKD_OK;
+}
+
+// Wide records of the inner nodes, numbered by an exclusive scan of the
+// inner-node flags in (depth-first) node order.  Needs row starts below 2^31.
+template <typename T, int D>
+int build_wide(ckd_tree *t, hipStream_t s) {
+    const int64_t nn = t->nnodes, nsets = t->nsets;
+    if (t->n >= INT32_MAX) return CKD_OK;  // stack traversals not offered
+    DevBuf flag, widx, scr;
+    int rc;
+    if ((rc = flag.alloc(sizeof(int64_t) * nn)) || (rc = widx.alloc(sizeof(int64_t) * (nn + 1))) ||
+        (rc = scr.alloc(scan_tmp_bytes(nn))))
+        return rc;
+    const NodeRec<T> *nodes = (const NodeRec<T> *)t->nodes;
+    const unsigned g = (unsigned)ceil_div(nn, 256);
+    k_inner_flags<T><<<g, 256, 0, s>>>(nodes, nn, flag.as<int64_t>());
+    if ((rc = scan_i64(flag.as<int64_t>(), widx.as<int64_t>(), nn, true, scr.p, s))) return rc;
+    int64_t ninner = 0;
+    CKD_HIP(hipMemcpyAsync(&ninner, widx.as<int64_t>() + nn, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    CKD_HIP(hipStreamSynchronize(s));
+    CKD_HIP(hipMalloc(&t->wide, wide_bytes<T, D>() * (size_t)(ninner > 0 ? ninner : 1)));
+    CKD_HIP(hipMalloc((void **)&t->wroot, sizeof(int2) * (size_t)nsets));
+    t->nwide = ninner;
+    k_fill_wide<T, D><<<g, 256, 0, s>>>(nodes, (const T *)t->dbox, t->range, widx.as<int64_t>(), nn,
+                                        (char *)t->wide);
+    k_root_pairs<T><<<(unsigned)ceil_div(nsets, 256), 256, 0, s>>>(nodes, t->range, widx.as<int64_t>(), nsets,
+                                                                    t->wroot);
+    CKD_HIP(hipGetLastError());
+    CKD_HIP(hipStreamSynchronize(s));  // the scratch buffers are freed on return
     return CKD_OK;
 }
 
@@ -765,6 +834,7 @@ int build_impl(ckd_tree *t, const T *in, hipStream_t s) {
             (const NodeRec<T> *)t->nodes, t->range, t->nnodes, (const T *)t->pts, (T *)t->dbox);
         CKD_HIP(hipGetLastError());
         if ((rc = renumber_dfs<T, D>(t, lvl_start, s))) return rc;
+        if ((rc = build_wide<T, D>(t, s))) return rc;
     }
     CKD_HIP(hipEventRecord(ev1, s));
     CKD_HIP(hipEventSynchronize(ev1));

@@ -14,6 +14,14 @@
 //   pts[n]          leaf-ordered points, AoS with row stride pstride(D)
 //                   elements (D=3 -> 4, so one 16/32-byte load per point).
 //   idx[n]          original 0-based index of each row, int32 or int64.
+//   wide[nwide]     one record per inner node, in depth-first order, for the
+//                   stack traversals: the tight boxes of both children
+//                   (lo0, hi0, lo1, hi1; 4*D values of T) followed by two
+//                   int32 pairs (a, b) saying where each child lives:
+//                   b < 0 -> inner node, a = its wide id; b >= 0 -> leaf
+//                   holding rows [a, a + b).  One load decides both children
+//                   and hands over a leaf's rows without touching nodes[].
+//   wroot[nsets]    the (a, b) pair of each set's root, same convention.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -42,6 +50,11 @@ struct Range {
 
 __host__ __device__ constexpr int pstride(int D) { return D == 3 ? 4 : (D <= 4 ? D : 8); }
 
+// Bytes of one wide record (a multiple of 16 for 4- and 8-byte T).
+template <typename T, int D>
+__host__ __device__ constexpr size_t wide_bytes() { return 4 * D * sizeof(T) + 4 * sizeof(int32_t); }
+__host__ __device__ constexpr size_t wide_bytes_rt(int D, int tsize) { return 4 * (size_t)D * tsize + 16; }
+
 // Device view of a tree, passed by value to kernels.
 template <typename T>
 struct TreeView {
@@ -51,6 +64,8 @@ struct TreeView {
     const Range *range;
     const T *pts;
     const void *idx;
+    const char *wide;     // wide records (nullptr when not built)
+    const int2 *wroot;    // root pair of each set
     int idx64;
     T L[CKD_MAX_DIM];  // box size per axis (periodic only)
 };
@@ -211,6 +226,11 @@ struct ckd_tree {
     void *idx = nullptr;     // int32 / int64 [n]
     int64_t nsets = 1;        // independent point sets (a forest); root of set r is node r
     int64_t *set_off = nullptr;  // device [nsets + 1]: set r holds input rows [set_off[r], set_off[r+1])
+    void *wide = nullptr;    // wide records of the inner nodes
+    int64_t nwide = 0;
+    int2 *wroot = nullptr;   // device [nsets] root pairs
+    int knn_kernel = -1;     // kNN kernel choice, -1 = automatic (ckd_knn_config)
+    int knn_stack = 0;       // stack depth of the stack kernel, 0 = automatic
     void *ws = nullptr;      // grow-only query workspace (calls on one tree must not overlap)
     size_t ws_bytes = 0;
     void *ws2 = nullptr;     // second workspace (retry lists)
@@ -227,6 +247,8 @@ TreeView<T> view(const ckd_tree *t) {
     v.range = t->range;
     v.pts = (const T *)t->pts;
     v.idx = t->idx;
+    v.wide = (const char *)t->wide;
+    v.wroot = t->wroot;
     v.idx64 = t->idx_bits == 64;
     for (int i = 0; i < CKD_MAX_DIM; i++) v.L[i] = (T)t->L[i];
     return v;

@@ -45,13 +45,38 @@ __global__ void k_fill_i64(int64_t *p, int64_t n, int64_t v) {
     if (i < n) p[i] = v;
 }
 
+// Per-wave reduction of the work counters, one atomic per counter.
+__device__ __forceinline__ void flush_stats(Counters c, int64_t nqw, int64_t *stats) {
+#pragma unroll
+    for (int m = WAVE / 2; m >= 1; m >>= 1) {
+        c.visited += __shfl_xor(c.visited, m, WAVE);
+        c.tested += __shfl_xor(c.tested, m, WAVE);
+        c.reruns += __shfl_xor(c.reruns, m, WAVE);
+        c.inserted += __shfl_xor(c.inserted, m, WAVE);
+        c.climbed += __shfl_xor(c.climbed, m, WAVE);
+        nqw += __shfl_xor(nqw, m, WAVE);
+    }
+    if ((threadIdx.x & (WAVE - 1)) == 0 && (nqw > 0 || c.visited > 0)) {
+        atomicAdd((unsigned long long *)&stats[0], (unsigned long long)c.visited);
+        atomicAdd((unsigned long long *)&stats[1], (unsigned long long)c.tested);
+        atomicAdd((unsigned long long *)&stats[2], (unsigned long long)c.reruns);
+        atomicAdd((unsigned long long *)&stats[3], (unsigned long long)nqw);
+        atomicAdd((unsigned long long *)&stats[4], (unsigned long long)c.inserted);
+        atomicAdd((unsigned long long *)&stats[5], (unsigned long long)c.climbed);
+    }
+}
+
+// One traversal per query (knn_one).  With nq_dev the query count is read
+// on the device (the retry list of the wave kernels; those queries are
+// not counted again in stats[3]).
 template <typename T, int D, bool PER, int KMAX>
-__global__ __launch_bounds__(QT) void k_knn(TreeView<T> tv, const T *queries, int64_t nq, int k, int sorted,
-                                            const float *seed, const int32_t *order, const int32_t *leaf_of,
-                                            T *dout, int64_t *iout, int64_t *stats) {
+__global__ __launch_bounds__(QT) void k_knn(TreeView<T> tv, const T *queries, int64_t nq, const int32_t *nq_dev,
+                                            int k, int sorted, const float *seed, const int32_t *order,
+                                            const int32_t *leaf_of, T *dout, int64_t *iout, int64_t *stats) {
     const int64_t t = (int64_t)blockIdx.x * QT + threadIdx.x;
+    const int64_t n = nq_dev ? (int64_t)*nq_dev : nq;
     Counters c;
-    if (t < nq) {
+    if (t < n) {
         const int64_t qi = order ? (int64_t)order[t] : t;
         T q[D];
 #pragma unroll
@@ -64,30 +89,11 @@ __global__ __launch_bounds__(QT) void k_knn(TreeView<T> tv, const T *queries, in
         }
         knn_one<T, D, PER, KMAX>(tv, q, leaf, k, sorted, seed2, dout + qi * k, iout + qi * k, 0, c);
     }
-    if (stats) {  // one atomic per wave and counter
-        int64_t nqw = t < nq ? 1 : 0;
-#pragma unroll
-        for (int m = WAVE / 2; m >= 1; m >>= 1) {
-            c.visited += __shfl_xor(c.visited, m, WAVE);
-            c.tested += __shfl_xor(c.tested, m, WAVE);
-            c.reruns += __shfl_xor(c.reruns, m, WAVE);
-            c.inserted += __shfl_xor(c.inserted, m, WAVE);
-            c.climbed += __shfl_xor(c.climbed, m, WAVE);
-            nqw += __shfl_xor(nqw, m, WAVE);
-        }
-        if ((threadIdx.x & (WAVE - 1)) == 0 && nqw > 0) {
-            atomicAdd((unsigned long long *)&stats[0], (unsigned long long)c.visited);
-            atomicAdd((unsigned long long *)&stats[1], (unsigned long long)c.tested);
-            atomicAdd((unsigned long long *)&stats[2], (unsigned long long)c.reruns);
-            atomicAdd((unsigned long long *)&stats[3], (unsigned long long)nqw);
-            atomicAdd((unsigned long long *)&stats[4], (unsigned long long)c.inserted);
-            atomicAdd((unsigned long long *)&stats[5], (unsigned long long)c.climbed);
-        }
-    }
+    if (stats) flush_stats(c, (t < n && !nq_dev) ? 1 : 0, stats);
 }
 
 // Wave-uniform variant (knn_wave): queries in leaf order, one traversal per
-// wave.
+// wave, parent-pointer walk (any tree depth).
 template <typename T, int D, bool PER, int KMAX>
 __global__ __launch_bounds__(QT, 3) void k_knn_w(TreeView<T> tv, const T *queries, int64_t nq, int k, int sorted,
                                               const float *seed, const int32_t *order, T *dout, int64_t *iout,
@@ -110,25 +116,42 @@ __global__ __launch_bounds__(QT, 3) void k_knn_w(TreeView<T> tv, const T *querie
                                                 0, c, buf_d, buf_s, threadIdx.x, QT);
     if (!done) retry[atomicAdd(nretry, 1)] = (int32_t)qi;
     if (stats) {
-        int64_t nqw = active ? 1 : 0;
         if (!active) c = Counters();
+        flush_stats(c, active ? 1 : 0, stats);
+    }
+}
+
+// Stack variant (knn_stack) over the wide records; SD >= tree depth.
+// Candidate buffering pays from k = 16 on (below, the short list update is
+// cheaper than the LDS round trip and the occupancy the buffer costs); 8
+// slots per lane in double precision keep the LDS within 4 blocks per CU.
+template <typename T, int D, bool PER, int KMAX, int SD>
+__global__ __launch_bounds__(QT, 3) void k_knn_s(TreeView<T> tv, int2 root, const T *queries, int64_t nq, int k,
+                                              int sorted, const float *seed, const int32_t *order, T *dout,
+                                              int64_t *iout, int64_t *stats, int32_t *retry, int32_t *nretry) {
+    constexpr int CB = KMAX < 16 ? 0 : (sizeof(T) == 4 ? 16 : 8);
+    __shared__ T buf_d[CB > 0 ? CB * QT : 1];
+    __shared__ int32_t buf_s[CB > 0 ? CB * QT : 1];
+    __shared__ StackEntry<T, D> stk[QT / WAVE][SD];
+    const int64_t t = (int64_t)blockIdx.x * QT + threadIdx.x;
+    Counters c;
+    const bool active = t < nq;
+    const int64_t qi = active ? (order ? (int64_t)order[t] : t) : 0;
+    T q[D];
 #pragma unroll
-        for (int m = WAVE / 2; m >= 1; m >>= 1) {
-            c.visited += __shfl_xor(c.visited, m, WAVE);
-            c.tested += __shfl_xor(c.tested, m, WAVE);
-            c.reruns += __shfl_xor(c.reruns, m, WAVE);
-            c.inserted += __shfl_xor(c.inserted, m, WAVE);
-            c.climbed += __shfl_xor(c.climbed, m, WAVE);
-            nqw += __shfl_xor(nqw, m, WAVE);
-        }
-        if ((threadIdx.x & (WAVE - 1)) == 0 && nqw > 0) {
-            atomicAdd((unsigned long long *)&stats[0], (unsigned long long)c.visited);
-            atomicAdd((unsigned long long *)&stats[1], (unsigned long long)c.tested);
-            atomicAdd((unsigned long long *)&stats[2], (unsigned long long)c.reruns);
-            atomicAdd((unsigned long long *)&stats[3], (unsigned long long)nqw);
-            atomicAdd((unsigned long long *)&stats[4], (unsigned long long)c.inserted);
-            atomicAdd((unsigned long long *)&stats[5], (unsigned long long)c.climbed);
-        }
+    for (int d = 0; d < D; d++) q[d] = active ? queries[qi * D + d] : T(0);
+    T seed2 = T(FLT_MAX);
+    if (seed && active) {
+        const T sr = (T)seed[qi];
+        seed2 = sr * sr;
+    }
+    const bool done = knn_stack<T, D, PER, KMAX, SD, CB>(tv, root, q, active, k, sorted, seed2, dout + qi * k,
+                                                          iout + qi * k, 0, c, buf_d, buf_s, threadIdx.x, QT,
+                                                          stk[threadIdx.x / WAVE]);
+    if (!done) retry[atomicAdd(nretry, 1)] = (int32_t)qi;
+    if (stats) {
+        if (!active) c = Counters();
+        flush_stats(c, active ? 1 : 0, stats);
     }
 }
 
@@ -248,15 +271,71 @@ __global__ __launch_bounds__(QT) void k_box(TreeView<T> tv, const T *boxes, int6
 
 // ------------------------------------------------------------ dispatch
 
-// 0 = one traversal per query (k_knn), 1 = one traversal per wave
-// (k_knn_w); CKD_KNN_KERNEL=lane|wave overrides the default.
-int knn_kernel_choice() {
-    static int choice = -1;
-    if (choice < 0) {
+// kNN kernel: CKD_KNN_LANE (one traversal per query, k_knn),
+// CKD_KNN_WAVE (one traversal per wave, parent pointers, k_knn_w) or
+// CKD_KNN_STACK (one traversal per wave over the wide records, k_knn_s).
+// The tree's setting (ckd_knn_config) wins over the environment variable
+// CKD_KNN_KERNEL=lane|wave|stack, which wins over the automatic choice.
+int knn_kernel_env() {
+    static int choice = -2;
+    if (choice == -2) {
         const char *e = getenv("CKD_KNN_KERNEL");
-        choice = (e && e[0] == 'w') ? 1 : 0;
+        choice = !e ? -1 : (e[0] == 'l' ? CKD_KNN_LANE : e[0] == 'w' ? CKD_KNN_WAVE : e[0] == 's' ? CKD_KNN_STACK : -1);
     }
     return choice;
+}
+
+template <typename T, int D>
+bool stack_ok(const ckd_tree *t, int sd) {
+    return D <= 3 && t->wide != nullptr && t->depth <= sd;
+}
+
+template <typename T, int D, bool PER, int KMAX>
+void launch_lane(const TreeView<T> &tv, unsigned g, const T *q, int64_t nq, const int32_t *nq_dev, int k,
+                 int sorted, const float *seed, const int32_t *order, const int32_t *leaf_of, T *dist, int64_t *idx,
+                 int64_t *stats, hipStream_t s) {
+    k_knn<T, D, PER, KMAX><<<g, QT, 0, s>>>(tv, q, nq, nq_dev, k, sorted, seed, order, leaf_of, dist, idx, stats);
+}
+
+template <typename T, int D, bool PER, int KMAX>
+int knn_launch_k(const ckd_tree *t, const TreeView<T> &tv, int kind, int sd, const T *q, int64_t nq, int k,
+                 int sorted, const float *seed, const int32_t *order, const int32_t *leaf_of, T *dist, int64_t *idx,
+                 int64_t *stats, hipStream_t s) {
+    const unsigned g = (unsigned)ceil_div(nq, QT);
+    if (kind == CKD_KNN_LANE) {
+        launch_lane<T, D, PER, KMAX>(tv, g, q, nq, nullptr, k, sorted, seed, order, leaf_of, dist, idx, stats, s);
+        CKD_HIP(hipGetLastError());
+        return CKD_OK;
+    }
+    // wave kernels: seeded queries whose seed held fewer than k points are
+    // listed in `retry` and searched again by the per-query kernel
+    int32_t *retry = nullptr, *nretry = nullptr;
+    if (seed) {
+        void *ws2 = nullptr;
+        int rc = workspace2(const_cast<ckd_tree *>(t), sizeof(int32_t) * (size_t)(nq + 64), &ws2, s);
+        if (rc) return rc;
+        nretry = (int32_t *)ws2;
+        retry = nretry + 64;
+        k_fill_i32<<<1, 64, 0, s>>>(nretry, 1, 0);
+    }
+    if (kind == CKD_KNN_STACK) {
+        if constexpr (D <= 3) {
+            const int2 root = make_int2(0, t->nnodes > 1 ? -1 : (int32_t)t->n);
+            if (sd <= 32)
+                k_knn_s<T, D, PER, KMAX, 32><<<g, QT, 0, s>>>(tv, root, q, nq, k, sorted, seed, order, dist, idx,
+                                                               stats, retry, nretry);
+            else
+                k_knn_s<T, D, PER, KMAX, 64><<<g, QT, 0, s>>>(tv, root, q, nq, k, sorted, seed, order, dist, idx,
+                                                               stats, retry, nretry);
+        }
+    } else {
+        k_knn_w<T, D, PER, KMAX><<<g, QT, 0, s>>>(tv, q, nq, k, sorted, seed, order, dist, idx, stats, retry, nretry);
+    }
+    CKD_HIP(hipGetLastError());
+    if (seed)
+        launch_lane<T, D, PER, KMAX>(tv, g, q, nq, nretry, k, sorted, nullptr, retry, nullptr, dist, idx, stats, s);
+    CKD_HIP(hipGetLastError());
+    return CKD_OK;
 }
 
 template <typename T, int D, bool PER>
@@ -265,6 +344,12 @@ int knn_launch(const ckd_tree *t, const T *q, int64_t nq, int k, int sorted, con
     const TreeView<T> tv = view<T>(t);
     const unsigned g = (unsigned)ceil_div(nq, QT);
     if (k <= 64 && t->n < INT32_MAX) {
+        // kernel and stack depth
+        int kind = t->knn_kernel >= 0 ? t->knn_kernel : knn_kernel_env();
+        int sd = t->knn_stack > 0 ? t->knn_stack : (t->depth <= 32 ? 32 : 64);
+        if (sd < t->depth) sd = 64;
+        if (kind < 0) kind = stack_ok<T, D>(t, 64) ? CKD_KNN_STACK : (k > 8 ? CKD_KNN_WAVE : CKD_KNN_LANE);
+        if (kind == CKD_KNN_STACK && !stack_ok<T, D>(t, sd)) kind = CKD_KNN_WAVE;  // same results, any depth
         const int32_t *order = nullptr, *leaf_of = nullptr;
         if (nq >= ORDER_MIN_QUERIES && nq < INT32_MAX && t->nnodes > 1) {
             // workspace: leaf_of, rank, order (int32 x nq), leaf counts and
@@ -285,43 +370,10 @@ int knn_launch(const ckd_tree *t, const T *q, int64_t nq, int k, int sorted, con
             order = ord;
             leaf_of = lo;
         }
-        if (knn_kernel_choice() == 1) {
-            // seeded queries whose seed held fewer than k points are listed
-            // in `retry` and searched again by the per-query kernel
-            int32_t *retry = nullptr, *nretry = nullptr;
-            if (seed) {
-                void *ws2 = nullptr;
-                int rc = workspace2(const_cast<ckd_tree *>(t), sizeof(int32_t) * (size_t)(nq + 64), &ws2, s);
-                if (rc) return rc;
-                nretry = (int32_t *)ws2;
-                retry = nretry + 64;
-                k_fill_i32<<<1, 64, 0, s>>>(nretry, 1, 0);
-            }
-            if (k <= 8) k_knn_w<T, D, PER, 8><<<g, QT, 0, s>>>(tv, q, nq, k, sorted, seed, order, dist, idx, stats, retry, nretry);
-            else if (k <= 16) k_knn_w<T, D, PER, 16><<<g, QT, 0, s>>>(tv, q, nq, k, sorted, seed, order, dist, idx, stats, retry, nretry);
-            else if (k <= 32) k_knn_w<T, D, PER, 32><<<g, QT, 0, s>>>(tv, q, nq, k, sorted, seed, order, dist, idx, stats, retry, nretry);
-            else k_knn_w<T, D, PER, 64><<<g, QT, 0, s>>>(tv, q, nq, k, sorted, seed, order, dist, idx, stats, retry, nretry);
-            CKD_HIP(hipGetLastError());
-            if (seed) {
-                int32_t h = 0;
-                CKD_HIP(hipMemcpyAsync(&h, nretry, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-                CKD_HIP(hipStreamSynchronize(s));
-                if (h > 0) {
-                    const unsigned g2 = (unsigned)ceil_div(h, QT);
-                    if (k <= 8) k_knn<T, D, PER, 8><<<g2, QT, 0, s>>>(tv, q, h, k, sorted, nullptr, retry, nullptr, dist, idx, stats);
-                    else if (k <= 16) k_knn<T, D, PER, 16><<<g2, QT, 0, s>>>(tv, q, h, k, sorted, nullptr, retry, nullptr, dist, idx, stats);
-                    else if (k <= 32) k_knn<T, D, PER, 32><<<g2, QT, 0, s>>>(tv, q, h, k, sorted, nullptr, retry, nullptr, dist, idx, stats);
-                    else k_knn<T, D, PER, 64><<<g2, QT, 0, s>>>(tv, q, h, k, sorted, nullptr, retry, nullptr, dist, idx, stats);
-                }
-            }
-        } else {
-            if (k <= 8) k_knn<T, D, PER, 8><<<g, QT, 0, s>>>(tv, q, nq, k, sorted, seed, order, leaf_of, dist, idx, stats);
-            else if (k <= 16) k_knn<T, D, PER, 16><<<g, QT, 0, s>>>(tv, q, nq, k, sorted, seed, order, leaf_of, dist, idx, stats);
-            else if (k <= 32) k_knn<T, D, PER, 32><<<g, QT, 0, s>>>(tv, q, nq, k, sorted, seed, order, leaf_of, dist, idx, stats);
-            else k_knn<T, D, PER, 64><<<g, QT, 0, s>>>(tv, q, nq, k, sorted, seed, order, leaf_of, dist, idx, stats);
-        }
-        CKD_HIP(hipGetLastError());
-        return CKD_OK;
+        if (k <= 8) return knn_launch_k<T, D, PER, 8>(t, tv, kind, sd, q, nq, k, sorted, seed, order, leaf_of, dist, idx, stats, s);
+        if (k <= 16) return knn_launch_k<T, D, PER, 16>(t, tv, kind, sd, q, nq, k, sorted, seed, order, leaf_of, dist, idx, stats, s);
+        if (k <= 32) return knn_launch_k<T, D, PER, 32>(t, tv, kind, sd, q, nq, k, sorted, seed, order, leaf_of, dist, idx, stats, s);
+        return knn_launch_k<T, D, PER, 64>(t, tv, kind, sd, q, nq, k, sorted, seed, order, leaf_of, dist, idx, stats, s);
     }
     k_knn_heap<T, D, PER><<<g, QT, 0, s>>>(tv, q, nq, k, dist, idx);
     CKD_HIP(hipGetLastError());

@@ -356,6 +356,223 @@ __device__ __forceinline__ bool knn_wave(const TreeView<T> &tv, const T (&q)[D],
     return true;
 }
 
+// ------------------------------------------------------------------
+// Stack traversal over the wide records (layout in ckd_internal.h).
+//
+// Same contract and result as knn_wave: the 64 queries of a wave share one
+// depth-first traversal, a subtree is entered when any lane's own bound
+// reaches its tight point box, each lane accepts with its own strict bound.
+// It differs in how the tree is walked: one wide record per inner node
+// gives both children's boxes and locations, so deciding a node costs one
+// (wave-uniform) load and a leaf's rows come with its parent's record; the
+// far child of a node whose two children are both needed is pushed on a
+// wave-uniform stack in LDS together with its box, and re-tested against
+// the bounds of that moment when popped.  The stack holds at most one entry
+// per level of the current path, so SD >= tree depth makes it overflow-free
+// (the launcher checks this).
+template <typename T, int D>
+__device__ __forceinline__ void load_wide(const char *wide, int32_t w, T (&b0)[2 * D], T (&b1)[2 * D], int4 &ref) {
+    const char *base = wide + (size_t)w * wide_bytes<T, D>();
+    const T *bx = (const T *)base;
+#pragma unroll
+    for (int d = 0; d < 2 * D; d++) {
+        b0[d] = bx[d];
+        b1[d] = bx[2 * D + d];
+    }
+    ref = *(const int4 *)(base + 4 * D * sizeof(T));
+}
+
+template <typename T, int D>
+struct StackEntry {
+    T box[2 * D];
+    int32_t a, b;
+};
+
+// Leaf scan shared by the stack kernels: rows [pb, pe), candidates buffered
+// per lane in LDS (CB > 0 slots per lane, slot j of lane tid at
+// j * nthreads + tid) or inserted directly (CB == 0).
+template <typename T, int D, bool PER, int KMAX, int CB>
+__device__ __forceinline__ void wave_scan_rows(const TreeView<T> &tv, int32_t pb, int32_t pe, const T (&q)[D],
+                                               bool run, T (&bd)[KMAX], int32_t (&bs)[KMAX], Counters &c,
+                                               T *buf_d, int32_t *buf_s, int tid, int nthreads) {
+    if constexpr (CB > 0) {
+        // leaves longer than CB rows go in chunks of CB, so a lane never
+        // buffers more than CB candidates; candidates are inserted in row
+        // order and re-tested against the current k-th distance
+#pragma unroll 1
+        for (int32_t cb = pb; cb < pe; cb += CB) {
+            const int32_t ce = cb + CB < pe ? cb + CB : pe;
+            const T bound = bd[KMAX - 1];
+            int cnt = 0;
+            for (int32_t p = cb; p < ce; p++) {
+                T x[D];
+                load_pt<T, D>(tv.pts, p, x);
+                const T d = dist2<T, D, PER>(x, q, tv.L);
+                if (run && d < bound) {
+                    buf_d[cnt * nthreads + tid] = d;
+                    buf_s[cnt * nthreads + tid] = p;
+                    cnt++;
+                }
+            }
+            const int jmax = __reduce_max_sync(~0ull, cnt);
+#pragma unroll 1
+            for (int j = 0; j < jmax; j++) {
+                if (j < cnt) {
+                    const T d = buf_d[j * nthreads + tid];
+                    if (d < bd[KMAX - 1]) {
+                        list_insert<T, KMAX>(bd, bs, d, buf_s[j * nthreads + tid]);
+                        c.inserted++;
+                    }
+                }
+            }
+        }
+    } else {
+        for (int32_t p = pb; p < pe; p++) {
+            T x[D];
+            load_pt<T, D>(tv.pts, p, x);
+            const T d = dist2<T, D, PER>(x, q, tv.L);
+            if (run && d < bd[KMAX - 1]) {
+                list_insert<T, KMAX>(bd, bs, d, p);
+                c.inserted++;
+            }
+        }
+    }
+    c.tested += pe - pb;
+}
+
+// Final rows of a kNN list: k (sqrt distance, index - idx_base) pairs,
+// equal distances in index order when `sorted`.
+template <typename T, int KMAX>
+__device__ __forceinline__ void knn_write_row(const TreeView<T> &tv, const T (&bd)[KMAX], const int32_t (&bs)[KMAX],
+                                              int k, int sorted, T *drow, int64_t *irow, int64_t idx_base) {
+    bool fix = false;
+    T pd = T(-1);
+    int64_t pi = 0;
+#pragma unroll
+    for (int j = 0; j < KMAX; j++) {
+        if (j >= KMAX - k) {
+            const int32_t s = bs[j];
+            const int64_t oi = s >= 0 ? load_idx(tv.idx, tv.idx64, s) - idx_base : (int64_t)CKD_EMPTY_SLOT;
+            const T dv = sqrt_rn<T>(bd[j]);
+            fix |= (dv == pd && oi < pi);
+            pd = dv;
+            pi = oi;
+            drow[j - (KMAX - k)] = dv;
+            irow[j - (KMAX - k)] = oi;
+        }
+    }
+    if (sorted && fix) {  // order equal distances by index (insertion sort, rare)
+        for (int a = 1; a < k; a++) {
+            const T dv = drow[a];
+            const int64_t iv = irow[a];
+            int b = a - 1;
+            while (b >= 0 && drow[b] == dv && irow[b] > iv) {
+                drow[b + 1] = drow[b];
+                irow[b + 1] = irow[b];
+                b--;
+            }
+            drow[b + 1] = dv;
+            irow[b + 1] = iv;
+        }
+    }
+}
+
+// Returns false for a seeded lane whose seed held fewer than k points (it
+// writes nothing then; the caller searches it again unseeded).  `stk` is
+// this wave's stack of SD entries in LDS.
+template <typename T, int D, bool PER, int KMAX, int SD, int CB>
+__device__ __forceinline__ bool knn_stack(const TreeView<T> &tv, int2 root, const T (&q)[D], bool active, int k,
+                                          int sorted, T seed2, T *drow, int64_t *irow, int64_t idx_base, Counters &c,
+                                          T *buf_d, int32_t *buf_s, int tid, int nthreads, StackEntry<T, D> *stk) {
+    T bd[KMAX];
+    int32_t bs[KMAX];
+    // near-first reference: mean of the active queries, made wave-uniform
+    T ref[D];
+    {
+        T n = active ? T(1) : T(0);
+#pragma unroll
+        for (int m = WAVE / 2; m >= 1; m >>= 1) n += __shfl_xor(n, m, WAVE);
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            T v = active ? q[d] : T(0);
+#pragma unroll
+            for (int m = WAVE / 2; m >= 1; m >>= 1) v += __shfl_xor(v, m, WAVE);
+            ref[d] = wave_uniform<T>(n > T(0) ? v / n : T(0));
+        }
+    }
+    const T init = seed2 < T(FLT_MAX) ? seed2 : T(FLT_MAX);
+    const bool run = active;
+#pragma unroll
+    for (int j = 0; j < KMAX; j++) {
+        bd[j] = j < KMAX - k ? T(-INFINITY) : init;
+        bs[j] = -1;
+    }
+    int sp = 0;
+    int32_t a = root.x, b = root.y;  // wave-uniform: the subtree to enter
+    for (;;) {
+        c.visited++;
+        bool descend = false;
+        if (b >= 0) {
+            wave_scan_rows<T, D, PER, KMAX, CB>(tv, a, a + b, q, run, bd, bs, c, buf_d, buf_s, tid, nthreads);
+        } else {
+            T b0[2 * D], b1[2 * D];
+            int4 r;
+            load_wide<T, D>(tv.wide, a, b0, b1, r);
+            const T bound = bd[KMAX - 1];
+            const bool n0 = __any(run && mindist2<T, D, PER>(b0, q, tv.L) < bound);
+            const bool n1 = __any(run && mindist2<T, D, PER>(b1, q, tv.L) < bound);
+            if (n0 && n1) {
+                // both: enter the one nearer the wave's mean query, push the other
+                const bool first1 = __builtin_amdgcn_readfirstlane(
+                    mindist2<T, D, PER>(b1, ref, tv.L) < mindist2<T, D, PER>(b0, ref, tv.L) ? 1 : 0);
+                StackEntry<T, D> &e = stk[sp];
+#pragma unroll
+                for (int d = 0; d < 2 * D; d++) e.box[d] = first1 ? b0[d] : b1[d];
+                e.a = first1 ? r.x : r.z;
+                e.b = first1 ? r.y : r.w;
+                sp++;
+                a = first1 ? r.z : r.x;
+                b = first1 ? r.w : r.y;
+                descend = true;
+            } else if (n0 || n1) {
+                a = n0 ? r.x : r.z;
+                b = n0 ? r.y : r.w;
+                descend = true;
+            }
+        }
+        if (descend) {  // the values are uniform; say so, for scalar branches and loads
+            a = __builtin_amdgcn_readfirstlane(a);
+            b = __builtin_amdgcn_readfirstlane(b);
+            continue;
+        }
+        // pop until a deferred subtree is still needed by some lane
+        bool found = false;
+        while (sp > 0) {
+            sp--;
+            const StackEntry<T, D> &e = stk[sp];
+            T bx[2 * D];
+#pragma unroll
+            for (int d = 0; d < 2 * D; d++) bx[d] = e.box[d];
+            const int32_t ea = __builtin_amdgcn_readfirstlane(e.a), eb = __builtin_amdgcn_readfirstlane(e.b);
+            c.climbed++;
+            if (__any(run && mindist2<T, D, PER>(bx, q, tv.L) < bd[KMAX - 1])) {
+                a = ea;
+                b = eb;
+                found = true;
+                break;
+            }
+        }
+        if (!found) break;
+    }
+    if (!active) return true;
+    if (bs[KMAX - 1] < 0 && init != T(FLT_MAX)) {
+        c.reruns++;
+        return false;
+    }
+    knn_write_row<T, KMAX>(tv, bd, bs, k, sorted, drow, irow, idx_base);
+    return true;
+}
+
 // Ball search below `root`, depth-first, left child first
 // (ball_search_recursive_bbox, cosmokdtree.f90:1076-1139): a subtree whose
 // point box is entirely farther than r is skipped, one entirely within r

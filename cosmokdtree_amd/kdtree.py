@@ -38,6 +38,7 @@ import ctypes as C
 import numpy as np
 import torch
 
+from . import _lib
 from ._lib import TreeInfo, check, lib
 
 __all__ = ["build_kdtree", "deallocate_kdtree", "knn_search", "ball_search", "box_search", "KDTree",
@@ -89,6 +90,15 @@ class KDTree(int):
             check(lib.ckd_free(self._ptr), "ckd_free")
             self._alive = False
             _REGISTRY.pop(int(self), None)
+
+    def knn_config(self, kernel: str = "auto", stack: int = 0):
+        """Pick the kNN kernel ("auto", "lane", "wave", "stack") and the
+        stack depth (0 = automatic, 32, 64).  Results do not depend on it."""
+        self._check_alive()
+        if kernel not in _lib.KNN_KERNELS:
+            raise ValueError(f"kernel must be one of {sorted(_lib.KNN_KERNELS)}")
+        check(lib.ckd_knn_config(self._ptr, _lib.KNN_KERNELS[kernel], int(stack)), "knn_config")
+        return self
 
     # ------------------------------------------------------------ device API
     def _queries(self, q) -> torch.Tensor:

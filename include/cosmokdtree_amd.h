@@ -106,10 +106,27 @@ int ckd_get_info(const ckd_tree *tree, ckd_tree_info *info);
  * without it, so results never depend on the seed.
  * stats [dev, 6 int64] or NULL: accumulates {nodes visited, leaf points
  * tested, queries re-run after a short seed, queries, candidates inserted
- * into a result list, parent steps climbed}.
+ * into a result list, parent steps climbed or deferred subtrees popped}.
+ * The wave kernels count per wave: nodes visited and pops are wave totals
+ * summed over the lanes.
  */
 int ckd_knn(const ckd_tree *tree, const void *queries, int64_t nq, int32_t k, int32_t sorted, void *dist,
             int64_t *idx, const float *seed_r, int64_t *stats, void *stream);
+
+#define CKD_KNN_AUTO (-1)
+#define CKD_KNN_LANE 0   /* one traversal per query */
+#define CKD_KNN_WAVE 1   /* the queries of a wave share one traversal (parent pointers, any depth) */
+#define CKD_KNN_STACK 2  /* shared traversal over wide records with an LDS stack (ndim <= 3) */
+
+/*
+ * Choose the kNN kernel of one tree (k <= 64).  No reference counterpart:
+ * the reference has a single CPU traversal.  kernel is one of CKD_KNN_*;
+ * stack_depth is 0 (automatic), 32 or 64, the LDS stack entries per wave of
+ * CKD_KNN_STACK, which is used only when ndim <= 3 and the tree depth fits
+ * the stack (otherwise CKD_KNN_WAVE runs).  Every choice returns identical
+ * results; only the speed differs.  Used by the autotuner.
+ */
+int ckd_knn_config(ckd_tree *tree, int32_t kernel, int32_t stack_depth);
 
 /*
  * Ball search, first pass: offsets [dev, nq+1 int64] receives the CSR row
