@@ -689,7 +689,8 @@ int build_impl(ckd_tree *t, const T *in, hipStream_t s) {
         (rc = I.alloc(sizeof(IdxT) * n)) || (rc = I2.alloc(sizeof(IdxT) * n)))
         return rc;
     // final arrays, owned by the tree
-    CKD_HIP(hipMalloc(&t->pts, sizeof(T) * PS * n));
+    CKD_HIP(hipMalloc(&t->pts, sizeof(T) * PS * (n + PTS_PAD)));
+    CKD_HIP(hipMemsetAsync((T *)t->pts + (size_t)PS * n, 0, sizeof(T) * PS * PTS_PAD, s));
     CKD_HIP(hipMalloc(&t->idx, sizeof(IdxT) * n));
 
     const int64_t nsets_ = t->nsets;

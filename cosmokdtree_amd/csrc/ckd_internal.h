@@ -12,7 +12,8 @@
 //                   [start, start+count) of the leaf-ordered arrays below,
 //                   for inner nodes too (a subtree is one contiguous range).
 //   pts[n]          leaf-ordered points, AoS with row stride pstride(D)
-//                   elements (D=3 -> 4, so one 16/32-byte load per point).
+//                   elements (D=3 -> 4, so one 16/32-byte load per point),
+//                   followed by PTS_PAD zero rows.
 //   idx[n]          original 0-based index of each row, int32 or int64.
 //   wide[nwide]     one record per inner node, in depth-first order, for the
 //                   stack traversals: the tight boxes of both children
@@ -50,6 +51,10 @@ struct Range {
 
 __host__ __device__ constexpr int pstride(int D) { return D == 3 ? 4 : (D <= 4 ? D : 8); }
 
+// Spare rows after the n leaf-ordered points (zero-filled), so that batched
+// row loads may run past the last leaf.
+constexpr int PTS_PAD = 8;
+
 // Bytes of one wide record (a multiple of 16 for 4- and 8-byte T).
 template <typename T, int D>
 __host__ __device__ constexpr size_t wide_bytes() { return 4 * D * sizeof(T) + 4 * sizeof(int32_t); }
@@ -80,6 +85,18 @@ int hip_fail(hipError_t e, const char *what, const char *file, int line);
     } while (0)
 
 // ---------------------------------------------------- device arithmetic
+// |x|, min, max as single instructions (source modifiers, v_min/v_max);
+// the operands are never NaN, and each returns one of its inputs exactly.
+__device__ __forceinline__ float absv(float x) { return __builtin_fabsf(x); }
+__device__ __forceinline__ double absv(double x) { return __builtin_fabs(x); }
+__device__ __forceinline__ float minv(float a, float b) { return __builtin_fminf(a, b); }
+__device__ __forceinline__ double minv(double a, double b) { return __builtin_fmin(a, b); }
+__device__ __forceinline__ float maxv(float a, float b) { return __builtin_fmaxf(a, b); }
+__device__ __forceinline__ double maxv(double a, double b) { return __builtin_fmax(a, b); }
+// median of three (lo <= hi): max(lo, min(x, hi)), one v_med3 in float
+__device__ __forceinline__ float med3v(float lo, float x, float hi) { return __builtin_amdgcn_fmed3f(lo, x, hi); }
+__device__ __forceinline__ double med3v(double lo, double x, double hi) { return maxv(lo, minv(x, hi)); }
+
 // Squared distance exactly as distance() of the reference
 // (cosmokdtree.f90:1441-1465): dx = p - q per axis, minimum image
 // min(|dx|, L - |dx|) when periodic, then ((0 + dx1*dx1) + dx2*dx2) + ...
@@ -91,40 +108,30 @@ __device__ __forceinline__ T dist2(const T *p, const T *q, const T *L) {
     for (int i = 0; i < D; i++) {
         T dx = p[i] - q[i];
         if (PER) {
-            T a = dx < T(0) ? -dx : dx;
-            T b = L[i] - a;
-            dx = b < a ? b : a;
+            const T a = absv(dx);
+            dx = minv(L[i] - a, a);
         }
         s = s + dx * dx;
     }
     return s;
 }
 
-// Lower bound of dist2 over every point of the cell [lo, hi].  Per axis the
-// bound is built from the same rounded differences the point distance uses,
-// so it never exceeds the distance computed for any point of the cell
-// (rounding is monotone).  Periodic: the nearer of the direct gap and the
-// wrapped gap L - (far face distance).
+// Lower bound of dist2 over every point of the box [lo, hi].  Per axis the
+// gap is max(lo - q, q - hi, 0), built from the same rounded differences
+// the point distance uses, so the bound never exceeds the distance computed
+// for any point of the box (rounding is monotone).  Periodic: the gap is
+// also capped by the wrapped gap L - (far face distance), where the far
+// face distance is -min(lo - q, q - hi) (a rounded difference negates
+// exactly).
 template <typename T, int D, bool PER>
 __device__ __forceinline__ T mindist2(const T *box, const T *q, const T *L) {
     T s = T(0);
 #pragma unroll
     for (int i = 0; i < D; i++) {
-        T lo = box[i], hi = box[D + i], d = T(0);
-        if (q[i] < lo) {
-            d = lo - q[i];
-            if (PER) {
-                T w = L[i] - (hi - q[i]);
-                d = w < d ? w : d;
-            }
-        } else if (q[i] > hi) {
-            d = q[i] - hi;
-            if (PER) {
-                T w = L[i] - (q[i] - lo);
-                d = w < d ? w : d;
-            }
-        }
-        d = d > T(0) ? d : T(0);
+        const T dl = box[i] - q[i], dh = q[i] - box[D + i];
+        T d;
+        if (PER) d = maxv(minv(maxv(dl, dh), L[i] + minv(dl, dh)), T(0));
+        else d = maxv(maxv(dl, dh), T(0));
         s = s + d * d;
     }
     return s;

@@ -2,7 +2,7 @@
 // GELU network on bf16 MFMA (v_mfma_f32_16x16x32_bf16, fp32 accumulation),
 // its backward pass and Adam.  No reference counterpart: the network only
 // seeds the kNN pruning bound; a query that finds fewer than k points inside
-// the seed falls back to the plain search (ckd_query.hip), so results never
+// the seed falls back to the plain search (ckd_query_impl.h), so results never
 // depend on it.
 //
 // Fragment layout of v_mfma_f32_16x16x32_bf16 in a wave of 64 lanes

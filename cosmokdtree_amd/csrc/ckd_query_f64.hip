@@ -1,0 +1,21 @@
+// ckd_query_f64.hip -- the double instantiation of the query kernels.
+#include "ckd_query_impl.h"
+
+namespace ckd {
+
+int query_knn_f64(const ckd_tree *t, const void *q, int64_t nq, int k, int sorted, const float *seed, void *dist,
+                  int64_t *idx, int64_t *stats, hipStream_t s) {
+    return knn_t<double>(t, q, nq, k, sorted, seed, dist, idx, stats, s);
+}
+
+int query_ball_f64(const ckd_tree *t, const void *q, int64_t nq, double r, int64_t *counts, const int64_t *off,
+                   void *dist, int64_t *idx, hipStream_t s) {
+    return ball_t<double>(t, q, nq, r, counts, off, dist, idx, s);
+}
+
+int query_box_f64(const ckd_tree *t, const void *b, int64_t nb, int64_t *counts, const int64_t *off, int64_t *idx,
+                  hipStream_t s) {
+    return box_t<double>(t, b, nb, counts, off, idx, s);
+}
+
+}  // namespace ckd

@@ -1,0 +1,456 @@
+// ckd_query_impl.h -- single-tree kNN, ball and box kernels and their host
+// dispatch, templated on the coordinate type; the traversals themselves are
+// in ckd_traverse.h.  Included once per precision (ckd_query_f32.hip,
+// ckd_query_f64.hip) so the two halves compile in parallel.
+#pragma once
+#include <cstdlib>
+
+#include "ckd_traverse.h"
+
+namespace ckd {
+
+namespace {
+
+constexpr int QT = 256;  // threads per block
+constexpr int64_t ORDER_MIN_QUERIES = 4096;  // below this, queries run in input order
+
+// Query ordering: each query's leaf, and its rank among the queries of
+// that leaf; a counting sort then lists the queries leaf by leaf so a wave
+// works on neighbouring queries.
+template <typename T, int D>
+__global__ __launch_bounds__(QT) void k_locate(TreeView<T> tv, const T *queries, int64_t nq, int32_t *leaf_of,
+                                               int32_t *rank, int64_t *leaf_cnt) {
+    const int64_t qi = (int64_t)blockIdx.x * QT + threadIdx.x;
+    if (qi >= nq) return;
+    T q[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) q[d] = queries[qi * D + d];
+    const int32_t leaf = find_leaf<T, D>(tv, q, 0);
+    leaf_of[qi] = leaf;
+    rank[qi] = (int32_t)atomicAdd((unsigned long long *)&leaf_cnt[leaf], 1ull);
+}
+
+__global__ void k_order(const int32_t *leaf_of, const int32_t *rank, const int64_t *leaf_off, int64_t nq,
+                        int32_t *order) {
+    const int64_t qi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (qi >= nq) return;
+    order[leaf_off[leaf_of[qi]] + rank[qi]] = (int32_t)qi;
+}
+
+__global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_fill_i64(int64_t *p, int64_t n, int64_t v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// Per-wave reduction of the work counters, one atomic per counter.
+__device__ __forceinline__ void flush_stats(Counters c, int64_t nqw, int64_t *stats) {
+#pragma unroll
+    for (int m = WAVE / 2; m >= 1; m >>= 1) {
+        c.visited += __shfl_xor(c.visited, m, WAVE);
+        c.tested += __shfl_xor(c.tested, m, WAVE);
+        c.reruns += __shfl_xor(c.reruns, m, WAVE);
+        c.inserted += __shfl_xor(c.inserted, m, WAVE);
+        c.climbed += __shfl_xor(c.climbed, m, WAVE);
+        nqw += __shfl_xor(nqw, m, WAVE);
+    }
+    if ((threadIdx.x & (WAVE - 1)) == 0 && (nqw > 0 || c.visited > 0)) {
+        atomicAdd((unsigned long long *)&stats[0], (unsigned long long)c.visited);
+        atomicAdd((unsigned long long *)&stats[1], (unsigned long long)c.tested);
+        atomicAdd((unsigned long long *)&stats[2], (unsigned long long)c.reruns);
+        atomicAdd((unsigned long long *)&stats[3], (unsigned long long)nqw);
+        atomicAdd((unsigned long long *)&stats[4], (unsigned long long)c.inserted);
+        atomicAdd((unsigned long long *)&stats[5], (unsigned long long)c.climbed);
+    }
+}
+
+// One traversal per query (knn_one).  With nq_dev the query count is read
+// on the device (the retry list of the wave kernels; those queries are
+// not counted again in stats[3]).
+template <typename T, int D, bool PER, int KMAX>
+__global__ __launch_bounds__(QT) void k_knn(TreeView<T> tv, const T *queries, int64_t nq, const int32_t *nq_dev,
+                                            int k, int sorted, const float *seed, const int32_t *order,
+                                            const int32_t *leaf_of, T *dout, int64_t *iout, int64_t *stats) {
+    const int64_t t = (int64_t)blockIdx.x * QT + threadIdx.x;
+    const int64_t n = nq_dev ? (int64_t)*nq_dev : nq;
+    Counters c;
+    if (t < n) {
+        const int64_t qi = order ? (int64_t)order[t] : t;
+        T q[D];
+#pragma unroll
+        for (int d = 0; d < D; d++) q[d] = queries[qi * D + d];
+        const int32_t leaf = leaf_of ? leaf_of[qi] : find_leaf<T, D>(tv, q, 0);
+        T seed2 = T(FLT_MAX);
+        if (seed) {
+            const T sr = (T)seed[qi];
+            seed2 = sr * sr;
+        }
+        knn_one<T, D, PER, KMAX>(tv, q, leaf, k, sorted, seed2, dout + qi * k, iout + qi * k, 0, c);
+    }
+    if (stats) flush_stats(c, (t < n && !nq_dev) ? 1 : 0, stats);
+}
+
+// Wave-uniform variant (knn_wave): queries in leaf order, one traversal per
+// wave, parent-pointer walk (any tree depth).
+template <typename T, int D, bool PER, int KMAX>
+__global__ __launch_bounds__(QT, 3) void k_knn_w(TreeView<T> tv, const T *queries, int64_t nq, int k, int sorted,
+                                              const float *seed, const int32_t *order, T *dout, int64_t *iout,
+                                              int64_t *stats, int32_t *retry, int32_t *nretry) {
+    __shared__ T buf_d[CBUF * QT];
+    __shared__ int32_t buf_s[CBUF * QT];
+    const int64_t t = (int64_t)blockIdx.x * QT + threadIdx.x;
+    Counters c;
+    const bool active = t < nq;
+    const int64_t qi = active ? (order ? (int64_t)order[t] : t) : 0;
+    T q[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) q[d] = active ? queries[qi * D + d] : T(0);
+    T seed2 = T(FLT_MAX);
+    if (seed && active) {
+        const T sr = (T)seed[qi];
+        seed2 = sr * sr;
+    }
+    const bool done = knn_wave<T, D, PER, KMAX>(tv, q, active, 0, k, sorted, seed2, dout + qi * k, iout + qi * k,
+                                                0, c, buf_d, buf_s, threadIdx.x, QT);
+    if (!done) retry[atomicAdd(nretry, 1)] = (int32_t)qi;
+    if (stats) {
+        if (!active) c = Counters();
+        flush_stats(c, active ? 1 : 0, stats);
+    }
+}
+
+// Stack variant (knn_stack) over the wide records; SD >= tree depth.
+// Candidate buffering pays from k = 16 on (below, the short list update is
+// cheaper than the LDS round trip and the occupancy the buffer costs); 8
+// slots per lane in double precision keep the LDS within 4 blocks per CU.
+template <typename T, int D, bool PER, int KMAX, int SD>
+__global__ __launch_bounds__(QT, 3) void k_knn_s(TreeView<T> tv, int2 root, const T *queries, int64_t nq, int k,
+                                              int sorted, const float *seed, const int32_t *order, T *dout,
+                                              int64_t *iout, int64_t *stats, int32_t *retry, int32_t *nretry) {
+    constexpr int CB = KMAX < 16 ? 0 : (sizeof(T) == 4 ? 16 : 8);
+    __shared__ T buf_d[CB > 0 ? CB * QT : 1];
+    __shared__ int32_t buf_s[CB > 0 ? CB * QT : 1];
+    __shared__ StackEntry<T, D> stk[QT / WAVE][SD];
+    const int64_t t = (int64_t)blockIdx.x * QT + threadIdx.x;
+    Counters c;
+    const bool active = t < nq;
+    const int64_t qi = active ? (order ? (int64_t)order[t] : t) : 0;
+    T q[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) q[d] = active ? queries[qi * D + d] : T(0);
+    T seed2 = T(FLT_MAX);
+    if (seed && active) {
+        const T sr = (T)seed[qi];
+        seed2 = sr * sr;
+    }
+    const bool done = knn_stack<T, D, PER, KMAX, SD, CB>(tv, root, q, active, k, sorted, seed2, dout + qi * k,
+                                                          iout + qi * k, 0, c, buf_d, buf_s, threadIdx.x, QT,
+                                                          stk[threadIdx.x / WAVE]);
+    if (!done) retry[atomicAdd(nretry, 1)] = (int32_t)qi;
+    if (stats) {
+        if (!active) c = Counters();
+        flush_stats(c, active ? 1 : 0, stats);
+    }
+}
+
+// kNN for k beyond the register list: the max-heap of the reference
+// (max_heap_insert, cosmokdtree.f90:859-905) kept in the output row, leaf
+// rows stored in the index row until the end.
+template <typename T, int D, bool PER>
+__global__ __launch_bounds__(QT) void k_knn_heap(TreeView<T> tv, const T *queries, int64_t nq, int k, T *dout,
+                                                 int64_t *iout) {
+    const int64_t qi = (int64_t)blockIdx.x * QT + threadIdx.x;
+    if (qi >= nq) return;
+    T q[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) q[d] = queries[qi * D + d];
+    T *hd = dout + qi * k;
+    int64_t *hi = iout + qi * k;
+    for (int j = 0; j < k; j++) {
+        hd[j] = T(FLT_MAX);
+        hi[j] = -1;
+    }
+    auto scan = [&](int32_t leaf) {
+        const Range r = tv.range[leaf];
+        T far = hd[0];
+        for (int64_t p = r.start; p < r.start + r.count; p++) {
+            T x[D];
+            load_pt<T, D>(tv.pts, p, x);
+            const T dc = dist2<T, D, PER>(x, q, tv.L);
+            if (dc < far) {
+                hd[0] = dc;
+                hi[0] = p;
+                int i = 0;
+                for (;;) {
+                    const int l = 2 * i + 1, rr = 2 * i + 2;
+                    int lg = i;
+                    if (l < k && hd[l] > hd[lg]) lg = l;
+                    if (rr < k && hd[rr] > hd[lg]) lg = rr;
+                    if (lg == i) break;
+                    const T td = hd[i];
+                    hd[i] = hd[lg];
+                    hd[lg] = td;
+                    const int64_t ti = hi[i];
+                    hi[i] = hi[lg];
+                    hi[lg] = ti;
+                    i = lg;
+                }
+                far = hd[0];
+            }
+        }
+    };
+    const int32_t leaf = find_leaf<T, D>(tv, q, 0);
+    scan(leaf);
+    int32_t from = leaf, cur = tv.nodes[leaf].parent;
+    while (cur >= 0) {
+        const NodeRec<T> nr = tv.nodes[cur];
+        const int32_t nearc = nr.child + (q[nr.axis] < nr.split ? 0 : 1);
+        bool moved = false;
+        if (from == nearc) {
+            const int32_t farc = nearc == nr.child ? nr.child + 1 : nr.child;
+            if (node_mindist<T, D, PER>(tv, farc, q) < hd[0]) {
+                int32_t n2 = farc;
+                for (;;) {
+                    const NodeRec<T> r2 = tv.nodes[n2];
+                    if (r2.axis < 0) {
+                        scan(n2);
+                        from = n2;
+                        cur = r2.parent;
+                        break;
+                    }
+                    const int32_t nc2 = r2.child + (q[r2.axis] < r2.split ? 0 : 1);
+                    if (node_mindist<T, D, PER>(tv, nc2, q) < hd[0]) {
+                        n2 = nc2;
+                        continue;
+                    }
+                    from = nc2;
+                    cur = n2;
+                    break;
+                }
+                moved = true;
+            }
+        }
+        if (!moved) {
+            from = cur;
+            cur = nr.parent;
+        }
+    }
+    for (int j = 0; j < k; j++) {
+        hd[j] = sqrt_rn<T>(hd[j]);
+        hi[j] = hi[j] >= 0 ? load_idx(tv.idx, tv.idx64, hi[j]) : (int64_t)CKD_EMPTY_SLOT;
+    }
+}
+
+// ---------------------------------------------------------------- ball / box
+template <typename T, int D, bool PER, bool FILL>
+__global__ __launch_bounds__(QT) void k_ball(TreeView<T> tv, const T *queries, int64_t nq, T r2, int64_t *counts,
+                                             const int64_t *offsets, T *dout, int64_t *iout) {
+    const int64_t qi = (int64_t)blockIdx.x * QT + threadIdx.x;
+    if (qi >= nq) return;
+    T q[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) q[d] = queries[qi * D + d];
+    const int64_t w = FILL ? offsets[qi] : 0;
+    const int64_t cnt = ball_one<T, D, PER, FILL>(tv, q, 0, r2, FILL ? dout + w : nullptr, FILL ? iout + w : nullptr, 0);
+    if (!FILL) counts[qi] = cnt;
+}
+
+template <typename T, int D, bool FILL>
+__global__ __launch_bounds__(QT) void k_box(TreeView<T> tv, const T *boxes, int64_t nb, int64_t *counts,
+                                            const int64_t *offsets, int64_t *iout) {
+    const int64_t bi = (int64_t)blockIdx.x * QT + threadIdx.x;
+    if (bi >= nb) return;
+    T bx[2 * D];
+#pragma unroll
+    for (int d = 0; d < 2 * D; d++) bx[d] = boxes[bi * 2 * D + d];
+    const int64_t cnt = box_one<T, D, FILL>(tv, bx, 0, FILL ? iout + offsets[bi] : nullptr, 0);
+    if (!FILL) counts[bi] = cnt;
+}
+
+// ------------------------------------------------------------ dispatch
+
+// kNN kernel: CKD_KNN_LANE (one traversal per query, k_knn),
+// CKD_KNN_WAVE (one traversal per wave, parent pointers, k_knn_w) or
+// CKD_KNN_STACK (one traversal per wave over the wide records, k_knn_s).
+// The tree's setting (ckd_knn_config) wins over the environment variable
+// CKD_KNN_KERNEL=lane|wave|stack, which wins over the automatic choice.
+int knn_kernel_env() {
+    static int choice = -2;
+    if (choice == -2) {
+        const char *e = getenv("CKD_KNN_KERNEL");
+        choice = !e ? -1 : (e[0] == 'l' ? CKD_KNN_LANE : e[0] == 'w' ? CKD_KNN_WAVE : e[0] == 's' ? CKD_KNN_STACK : -1);
+    }
+    return choice;
+}
+
+template <typename T, int D>
+bool stack_ok(const ckd_tree *t, int sd) {
+    return D <= 3 && t->wide != nullptr && t->depth <= sd;
+}
+
+template <typename T, int D, bool PER, int KMAX>
+void launch_lane(const TreeView<T> &tv, unsigned g, const T *q, int64_t nq, const int32_t *nq_dev, int k,
+                 int sorted, const float *seed, const int32_t *order, const int32_t *leaf_of, T *dist, int64_t *idx,
+                 int64_t *stats, hipStream_t s) {
+    k_knn<T, D, PER, KMAX><<<g, QT, 0, s>>>(tv, q, nq, nq_dev, k, sorted, seed, order, leaf_of, dist, idx, stats);
+}
+
+template <typename T, int D, bool PER, int KMAX>
+int knn_launch_k(const ckd_tree *t, const TreeView<T> &tv, int kind, int sd, const T *q, int64_t nq, int k,
+                 int sorted, const float *seed, const int32_t *order, const int32_t *leaf_of, T *dist, int64_t *idx,
+                 int64_t *stats, hipStream_t s) {
+    const unsigned g = (unsigned)ceil_div(nq, QT);
+    if (kind == CKD_KNN_LANE) {
+        launch_lane<T, D, PER, KMAX>(tv, g, q, nq, nullptr, k, sorted, seed, order, leaf_of, dist, idx, stats, s);
+        CKD_HIP(hipGetLastError());
+        return CKD_OK;
+    }
+    // wave kernels: seeded queries whose seed held fewer than k points are
+    // listed in `retry` and searched again by the per-query kernel
+    int32_t *retry = nullptr, *nretry = nullptr;
+    if (seed) {
+        void *ws2 = nullptr;
+        int rc = workspace2(const_cast<ckd_tree *>(t), sizeof(int32_t) * (size_t)(nq + 64), &ws2, s);
+        if (rc) return rc;
+        nretry = (int32_t *)ws2;
+        retry = nretry + 64;
+        k_fill_i32<<<1, 64, 0, s>>>(nretry, 1, 0);
+    }
+    if (kind == CKD_KNN_STACK) {
+        if constexpr (D <= 3) {
+            const int2 root = make_int2(0, t->nnodes > 1 ? -1 : (int32_t)t->n);
+            if (sd <= 32)
+                k_knn_s<T, D, PER, KMAX, 32><<<g, QT, 0, s>>>(tv, root, q, nq, k, sorted, seed, order, dist, idx,
+                                                               stats, retry, nretry);
+            else
+                k_knn_s<T, D, PER, KMAX, 64><<<g, QT, 0, s>>>(tv, root, q, nq, k, sorted, seed, order, dist, idx,
+                                                               stats, retry, nretry);
+        }
+    } else if constexpr (D <= 3) {
+        k_knn_w<T, D, PER, KMAX><<<g, QT, 0, s>>>(tv, q, nq, k, sorted, seed, order, dist, idx, stats, retry, nretry);
+    }
+    CKD_HIP(hipGetLastError());
+    if (seed)
+        launch_lane<T, D, PER, KMAX>(tv, g, q, nq, nretry, k, sorted, nullptr, retry, nullptr, dist, idx, stats, s);
+    CKD_HIP(hipGetLastError());
+    return CKD_OK;
+}
+
+template <typename T, int D, bool PER>
+int knn_launch(const ckd_tree *t, const T *q, int64_t nq, int k, int sorted, const float *seed, T *dist,
+               int64_t *idx, int64_t *stats, hipStream_t s) {
+    const TreeView<T> tv = view<T>(t);
+    const unsigned g = (unsigned)ceil_div(nq, QT);
+    if (k <= 64 && t->n < INT32_MAX) {
+        // kernel and stack depth
+        int kind = t->knn_kernel >= 0 ? t->knn_kernel : knn_kernel_env();
+        int sd = t->knn_stack > 0 ? t->knn_stack : (t->depth <= 32 ? 32 : 64);
+        if (sd < t->depth) sd = 64;
+        if (kind < 0) kind = stack_ok<T, D>(t, 64) ? CKD_KNN_STACK : (k > 8 ? CKD_KNN_WAVE : CKD_KNN_LANE);
+        if (kind == CKD_KNN_STACK && !stack_ok<T, D>(t, sd)) kind = CKD_KNN_WAVE;  // same results, any depth
+        if (D > 3) kind = CKD_KNN_LANE;  // the wave kernels are built for ndim <= 3
+        const int32_t *order = nullptr, *leaf_of = nullptr;
+        if (nq >= ORDER_MIN_QUERIES && nq < INT32_MAX && t->nnodes > 1) {
+            // workspace: leaf_of, rank, order (int32 x nq), leaf counts and
+            // offsets (int64 x nnodes+1), scan scratch
+            const size_t a = 3 * sizeof(int32_t) * (size_t)nq, b = sizeof(int64_t) * (size_t)(t->nnodes + 1);
+            void *ws = nullptr;
+            int rc = workspace(const_cast<ckd_tree *>(t), a + 2 * b + scan_tmp_bytes(t->nnodes) + 256, &ws, s);
+            if (rc) return rc;
+            int32_t *lo = (int32_t *)ws, *rk = lo + nq, *ord = rk + nq;
+            int64_t *cnt = (int64_t *)((char *)ws + ((a + 255) & ~(size_t)255));
+            int64_t *off = cnt + t->nnodes + 1;
+            void *scr = off + t->nnodes + 1;
+            k_fill_i64<<<(unsigned)ceil_div(t->nnodes, 256), 256, 0, s>>>(cnt, t->nnodes, 0);
+            k_locate<T, D><<<g, QT, 0, s>>>(tv, q, nq, lo, rk, cnt);
+            if ((rc = scan_i64(cnt, off, t->nnodes, false, scr, s))) return rc;
+            k_order<<<(unsigned)ceil_div(nq, 256), 256, 0, s>>>(lo, rk, off, nq, ord);
+            CKD_HIP(hipGetLastError());
+            order = ord;
+            leaf_of = lo;
+        }
+        if (k <= 8) return knn_launch_k<T, D, PER, 8>(t, tv, kind, sd, q, nq, k, sorted, seed, order, leaf_of, dist, idx, stats, s);
+        if (k <= 16) return knn_launch_k<T, D, PER, 16>(t, tv, kind, sd, q, nq, k, sorted, seed, order, leaf_of, dist, idx, stats, s);
+        if (k <= 32) return knn_launch_k<T, D, PER, 32>(t, tv, kind, sd, q, nq, k, sorted, seed, order, leaf_of, dist, idx, stats, s);
+        return knn_launch_k<T, D, PER, 64>(t, tv, kind, sd, q, nq, k, sorted, seed, order, leaf_of, dist, idx, stats, s);
+    }
+    k_knn_heap<T, D, PER><<<g, QT, 0, s>>>(tv, q, nq, k, dist, idx);
+    CKD_HIP(hipGetLastError());
+    if (sorted) return segmented_sort<T>(dist, idx, nullptr, nq, k, s);
+    return CKD_OK;
+}
+
+template <typename T, int D>
+int ball_launch(const ckd_tree *t, const T *q, int64_t nq, T r2, int64_t *counts, const int64_t *off, T *dist,
+                int64_t *idx, hipStream_t s) {
+    const TreeView<T> tv = view<T>(t);
+    const unsigned g = (unsigned)ceil_div(nq, QT);
+    const bool fill = off != nullptr;
+    if (t->periodic) {
+        if (fill) k_ball<T, D, true, true><<<g, QT, 0, s>>>(tv, q, nq, r2, counts, off, dist, idx);
+        else k_ball<T, D, true, false><<<g, QT, 0, s>>>(tv, q, nq, r2, counts, off, dist, idx);
+    } else {
+        if (fill) k_ball<T, D, false, true><<<g, QT, 0, s>>>(tv, q, nq, r2, counts, off, dist, idx);
+        else k_ball<T, D, false, false><<<g, QT, 0, s>>>(tv, q, nq, r2, counts, off, dist, idx);
+    }
+    CKD_HIP(hipGetLastError());
+    return CKD_OK;
+}
+
+template <typename T, int D>
+int box_launch(const ckd_tree *t, const T *b, int64_t nb, int64_t *counts, const int64_t *off, int64_t *idx,
+               hipStream_t s) {
+    const TreeView<T> tv = view<T>(t);
+    const unsigned g = (unsigned)ceil_div(nb, QT);
+    if (off) k_box<T, D, true><<<g, QT, 0, s>>>(tv, b, nb, counts, off, idx);
+    else k_box<T, D, false><<<g, QT, 0, s>>>(tv, b, nb, counts, off, idx);
+    CKD_HIP(hipGetLastError());
+    return CKD_OK;
+}
+
+#define CKD_DIM_SWITCH(ndim, CALL)            \
+    switch (ndim) {                           \
+        case 1: { constexpr int D_ = 1; return CALL; } \
+        case 2: { constexpr int D_ = 2; return CALL; } \
+        case 3: { constexpr int D_ = 3; return CALL; } \
+        case 4: { constexpr int D_ = 4; return CALL; } \
+        case 5: { constexpr int D_ = 5; return CALL; } \
+        case 6: { constexpr int D_ = 6; return CALL; } \
+        case 7: { constexpr int D_ = 7; return CALL; } \
+        case 8: { constexpr int D_ = 8; return CALL; } \
+    }
+
+template <typename T>
+int knn_t(const ckd_tree *t, const void *q, int64_t nq, int k, int sorted, const float *seed, void *dist,
+          int64_t *idx, int64_t *stats, hipStream_t s) {
+    if (t->periodic) {
+        CKD_DIM_SWITCH(t->ndim, (knn_launch<T, D_, true>(t, (const T *)q, nq, k, sorted, seed, (T *)dist, idx, stats, s)))
+    } else {
+        CKD_DIM_SWITCH(t->ndim, (knn_launch<T, D_, false>(t, (const T *)q, nq, k, sorted, seed, (T *)dist, idx, stats, s)))
+    }
+    return CKD_ERR_ARG;
+}
+
+template <typename T>
+int ball_t(const ckd_tree *t, const void *q, int64_t nq, double r, int64_t *counts, const int64_t *off, void *dist,
+           int64_t *idx, hipStream_t s) {
+    const T rr = (T)r;   // radius is a real(prec) dummy (cosmokdtree.f90:1019)
+    const T r2 = rr * rr;
+    CKD_DIM_SWITCH(t->ndim, (ball_launch<T, D_>(t, (const T *)q, nq, r2, counts, off, (T *)dist, idx, s)))
+    return CKD_ERR_ARG;
+}
+
+template <typename T>
+int box_t(const ckd_tree *t, const void *b, int64_t nb, int64_t *counts, const int64_t *off, int64_t *idx,
+          hipStream_t s) {
+    CKD_DIM_SWITCH(t->ndim, (box_launch<T, D_>(t, (const T *)b, nb, counts, off, idx, s)))
+    return CKD_ERR_ARG;
+}
+
+}  // namespace
+
+}  // namespace ckd

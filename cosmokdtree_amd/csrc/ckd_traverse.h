@@ -1,5 +1,5 @@
 // ckd_traverse.h -- per-query traversals shared by the single-tree kernels
-// (ckd_query.hip) and the batched persistent kernel (ckd_batch.hip).
+// (ckd_query_impl.h) and the batched persistent kernel (ckd_batch.hip).
 //
 // One thread per query.  Traversals are stack-free: a node's parent pointer
 // replaces the recursion stack of the reference, so any tree depth
@@ -55,18 +55,20 @@ __device__ __forceinline__ T node_mindist(const TreeView<T> &tv, int32_t node, c
     return mindist2<T, D, PER>(b, q, tv.L);
 }
 
+// Insert (d, s), d < bd[KMAX-1], into the ascending list; the last entry
+// drops out and d goes after the entries equal to it.  Slot j becomes the
+// median of (old j-1, d, old j); the rows follow the same choice.
 template <typename T, int KMAX>
 __device__ __forceinline__ void list_insert(T (&bd)[KMAX], int32_t (&bs)[KMAX], T d, int32_t s) {
 #pragma unroll
     for (int j = KMAX - 1; j > 0; --j) {
-        const bool sh = d < bd[j - 1];
-        const bool here = !sh && d < bd[j];
-        bd[j] = sh ? bd[j - 1] : (here ? d : bd[j]);
-        bs[j] = sh ? bs[j - 1] : (here ? s : bs[j]);
+        const bool below = d < bd[j - 1];
+        const bool here = d < bd[j];
+        bs[j] = below ? bs[j - 1] : (here ? s : bs[j]);
+        bd[j] = med3v(bd[j - 1], d, bd[j]);
     }
-    const bool here = d < bd[0];
-    bd[0] = here ? d : bd[0];
-    bs[0] = here ? s : bs[0];
+    bs[0] = d < bd[0] ? s : bs[0];
+    bd[0] = minv(bd[0], d);
 }
 
 // The leaf a point falls in below `root`: the nearer child at every level
@@ -388,6 +390,21 @@ struct StackEntry {
     int32_t a, b;
 };
 
+// PB consecutive rows from a wave-uniform row p on, as one contiguous
+// block, so all PB loads are issued (and merged) before the first wait
+// instead of one load latency per row.  Rows past the leaf may be read
+// (the caller masks them): the point array carries PTS_PAD spare rows.
+template <typename T, int D, int PB>
+__device__ __forceinline__ void load_rows(const T *pts, int32_t p, T (&x)[PB][D]) {
+    static_assert(PB <= PTS_PAD, "row batch longer than the padding");
+    constexpr int PS = pstride(D);
+    const T *src = pts + (int64_t)p * PS;
+#pragma unroll
+    for (int j = 0; j < PB; j++)
+#pragma unroll
+        for (int d = 0; d < D; d++) x[j][d] = src[j * PS + d];
+}
+
 // Leaf scan shared by the stack kernels: rows [pb, pe), candidates buffered
 // per lane in LDS (CB > 0 slots per lane, slot j of lane tid at
 // j * nthreads + tid) or inserted directly (CB == 0).
@@ -395,6 +412,7 @@ template <typename T, int D, bool PER, int KMAX, int CB>
 __device__ __forceinline__ void wave_scan_rows(const TreeView<T> &tv, int32_t pb, int32_t pe, const T (&q)[D],
                                                bool run, T (&bd)[KMAX], int32_t (&bs)[KMAX], Counters &c,
                                                T *buf_d, int32_t *buf_s, int tid, int nthreads) {
+    constexpr int PB = sizeof(T) == 4 ? 8 : 4;  // rows loaded per batch (CB is a multiple)
     if constexpr (CB > 0) {
         // leaves longer than CB rows go in chunks of CB, so a lane never
         // buffers more than CB candidates; candidates are inserted in row
@@ -404,14 +422,22 @@ __device__ __forceinline__ void wave_scan_rows(const TreeView<T> &tv, int32_t pb
             const int32_t ce = cb + CB < pe ? cb + CB : pe;
             const T bound = bd[KMAX - 1];
             int cnt = 0;
-            for (int32_t p = cb; p < ce; p++) {
-                T x[D];
-                load_pt<T, D>(tv.pts, p, x);
-                const T d = dist2<T, D, PER>(x, q, tv.L);
-                if (run && d < bound) {
-                    buf_d[cnt * nthreads + tid] = d;
-                    buf_s[cnt * nthreads + tid] = p;
-                    cnt++;
+            T *wd = buf_d + tid;  // this lane's next free slot
+            int32_t *ws = buf_s + tid;
+#pragma unroll 1
+            for (int32_t p = cb; p < ce; p += PB) {
+                T x[PB][D];
+                load_rows<T, D, PB>(tv.pts, p, x);
+#pragma unroll
+                for (int j = 0; j < PB; j++) {
+                    const T d = dist2<T, D, PER>(x[j], q, tv.L);
+                    if (run && p + j < ce && d < bound) {
+                        *wd = d;
+                        *ws = p + j;
+                        wd += nthreads;
+                        ws += nthreads;
+                        cnt++;
+                    }
                 }
             }
             const int jmax = __reduce_max_sync(~0ull, cnt);
@@ -427,13 +453,17 @@ __device__ __forceinline__ void wave_scan_rows(const TreeView<T> &tv, int32_t pb
             }
         }
     } else {
-        for (int32_t p = pb; p < pe; p++) {
-            T x[D];
-            load_pt<T, D>(tv.pts, p, x);
-            const T d = dist2<T, D, PER>(x, q, tv.L);
-            if (run && d < bd[KMAX - 1]) {
-                list_insert<T, KMAX>(bd, bs, d, p);
-                c.inserted++;
+#pragma unroll 1
+        for (int32_t p = pb; p < pe; p += PB) {
+            T x[PB][D];
+            load_rows<T, D, PB>(tv.pts, p, x);
+#pragma unroll
+            for (int j = 0; j < PB; j++) {
+                const T d = dist2<T, D, PER>(x[j], q, tv.L);
+                if (run && p + j < pe && d < bd[KMAX - 1]) {
+                    list_insert<T, KMAX>(bd, bs, d, p + j);
+                    c.inserted++;
+                }
             }
         }
     }
@@ -477,39 +507,35 @@ __device__ __forceinline__ void knn_write_row(const TreeView<T> &tv, const T (&b
     }
 }
 
-// Returns false for a seeded lane whose seed held fewer than k points (it
-// writes nothing then; the caller searches it again unseeded).  `stk` is
-// this wave's stack of SD entries in LDS.
-template <typename T, int D, bool PER, int KMAX, int SD, int CB>
-__device__ __forceinline__ bool knn_stack(const TreeView<T> &tv, int2 root, const T (&q)[D], bool active, int k,
-                                          int sorted, T seed2, T *drow, int64_t *irow, int64_t idx_base, Counters &c,
-                                          T *buf_d, int32_t *buf_s, int tid, int nthreads, StackEntry<T, D> *stk) {
-    T bd[KMAX];
-    int32_t bs[KMAX];
-    // near-first reference: mean of the active queries, made wave-uniform
-    T ref[D];
-    {
-        T n = active ? T(1) : T(0);
+// Periodic shortcut of the stack walk.  A point whose minimum image wraps
+// on some axis (L - |dx| < |dx|) is at least w_i = L - |dx_i| >= L/2 - |q_i|
+// away on that axis, so its distance can never be accepted once the
+// lane's bound is at most m^2, m = min_i (L/2 - |q_i|) minus a margin that
+// covers the rounding of |dx| and L - |dx| (1e-3 L, far above it).  When
+// every lane of the wave is there, the walk continues with the plain
+// (non-periodic) formulas: for the points that do not wrap they give the
+// same bits, the ones that wrap are rejected either way, and the plain box
+// bound is a lower bound of every acceptable point's distance.  Bounds
+// only shrink, so the switch is made once.
+template <typename T, int D>
+__device__ __forceinline__ T wrap_free_bound(const T (&q)[D], const T *L) {
+    T m = T(FLT_MAX);
 #pragma unroll
-        for (int m = WAVE / 2; m >= 1; m >>= 1) n += __shfl_xor(n, m, WAVE);
-#pragma unroll
-        for (int d = 0; d < D; d++) {
-            T v = active ? q[d] : T(0);
-#pragma unroll
-            for (int m = WAVE / 2; m >= 1; m >>= 1) v += __shfl_xor(v, m, WAVE);
-            ref[d] = wave_uniform<T>(n > T(0) ? v / n : T(0));
-        }
-    }
-    const T init = seed2 < T(FLT_MAX) ? seed2 : T(FLT_MAX);
-    const bool run = active;
-#pragma unroll
-    for (int j = 0; j < KMAX; j++) {
-        bd[j] = j < KMAX - k ? T(-INFINITY) : init;
-        bs[j] = -1;
-    }
-    int sp = 0;
-    int32_t a = root.x, b = root.y;  // wave-uniform: the subtree to enter
+    for (int d = 0; d < D; d++) m = minv(m, L[d] * T(0.5) - absv(q[d]) - L[d] * T(1e-3));
+    m = maxv(m, T(0));
+    return m * m;
+}
+
+// The walk of knn_stack from the item (a, b) with sp stacked entries.
+// Returns true when the traversal is complete; with CHECK it instead
+// returns false, (a, b) not yet processed, as soon as every running lane's
+// bound is within its wrap-free bound `safe2`.
+template <typename T, int D, bool PER, int KMAX, int SD, int CB, bool CHECK>
+__device__ __forceinline__ bool stack_walk(const TreeView<T> &tv, const T (&q)[D], bool run, T safe2, T (&bd)[KMAX],
+                                           int32_t (&bs)[KMAX], int32_t &a, int32_t &b, int &sp, Counters &c,
+                                           T *buf_d, int32_t *buf_s, int tid, int nthreads, StackEntry<T, D> *stk) {
     for (;;) {
+        if (CHECK && !__any(run && !(safe2 >= bd[KMAX - 1]))) return false;
         c.visited++;
         bool descend = false;
         if (b >= 0) {
@@ -519,12 +545,12 @@ __device__ __forceinline__ bool knn_stack(const TreeView<T> &tv, int2 root, cons
             int4 r;
             load_wide<T, D>(tv.wide, a, b0, b1, r);
             const T bound = bd[KMAX - 1];
-            const bool n0 = __any(run && mindist2<T, D, PER>(b0, q, tv.L) < bound);
-            const bool n1 = __any(run && mindist2<T, D, PER>(b1, q, tv.L) < bound);
+            const T m0 = mindist2<T, D, PER>(b0, q, tv.L), m1 = mindist2<T, D, PER>(b1, q, tv.L);
+            const bool n0 = __any(run && m0 < bound);
+            const bool n1 = __any(run && m1 < bound);
             if (n0 && n1) {
-                // both: enter the one nearer the wave's mean query, push the other
-                const bool first1 = __builtin_amdgcn_readfirstlane(
-                    mindist2<T, D, PER>(b1, ref, tv.L) < mindist2<T, D, PER>(b0, ref, tv.L) ? 1 : 0);
+                // both: enter first the child more lanes are nearer to, push the other
+                const bool first1 = __popcll(__ballot(run && m1 < m0)) > __popcll(__ballot(run && m0 < m1));
                 StackEntry<T, D> &e = stk[sp];
 #pragma unroll
                 for (int d = 0; d < 2 * D; d++) e.box[d] = first1 ? b0[d] : b1[d];
@@ -562,7 +588,37 @@ __device__ __forceinline__ bool knn_stack(const TreeView<T> &tv, int2 root, cons
                 break;
             }
         }
-        if (!found) break;
+        if (!found) return true;
+    }
+}
+
+// Returns false for a seeded lane whose seed held fewer than k points (it
+// writes nothing then; the caller searches it again unseeded).  `stk` is
+// this wave's stack of SD entries in LDS.
+template <typename T, int D, bool PER, int KMAX, int SD, int CB>
+__device__ __forceinline__ bool knn_stack(const TreeView<T> &tv, int2 root, const T (&q)[D], bool active, int k,
+                                          int sorted, T seed2, T *drow, int64_t *irow, int64_t idx_base, Counters &c,
+                                          T *buf_d, int32_t *buf_s, int tid, int nthreads, StackEntry<T, D> *stk) {
+    T bd[KMAX];
+    int32_t bs[KMAX];
+    const T init = seed2 < T(FLT_MAX) ? seed2 : T(FLT_MAX);
+    const bool run = active;
+#pragma unroll
+    for (int j = 0; j < KMAX; j++) {
+        bd[j] = j < KMAX - k ? T(-INFINITY) : init;
+        bs[j] = -1;
+    }
+    int sp = 0;
+    int32_t a = root.x, b = root.y;  // wave-uniform: the subtree to enter
+    if constexpr (PER) {
+        const T safe2 = wrap_free_bound<T, D>(q, tv.L);
+        if (!stack_walk<T, D, true, KMAX, SD, CB, true>(tv, q, run, safe2, bd, bs, a, b, sp, c, buf_d, buf_s, tid,
+                                                        nthreads, stk))
+            stack_walk<T, D, false, KMAX, SD, CB, false>(tv, q, run, safe2, bd, bs, a, b, sp, c, buf_d, buf_s, tid,
+                                                         nthreads, stk);
+    } else {
+        stack_walk<T, D, false, KMAX, SD, CB, false>(tv, q, run, T(0), bd, bs, a, b, sp, c, buf_d, buf_s, tid,
+                                                     nthreads, stk);
     }
     if (!active) return true;
     if (bs[KMAX - 1] < 0 && init != T(FLT_MAX)) {
