@@ -128,7 +128,7 @@ def main():
     # ---- kNN steps
     for _ in range(args.warmup):
         d, i = tree.knn(P, K, sorted=True)
-    del d, i
+        del d, i
     torch.cuda.synchronize()
     stats = torch.zeros(6, dtype=torch.int64, device=dev)
     d, i = tree.knn(P[:1 << 20], K, sorted=True, stats=stats)  # visit counts on a 1M sample (untimed)
@@ -136,11 +136,14 @@ def main():
     del d, i
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    tree.knn_timing(True)  # HIP events around each traversal kernel launch, on the launch stream
     e0.record()
     for _ in range(args.steps):
         d, i = tree.knn(P, K, sorted=True)
     e1.record()
     torch.cuda.synchronize()
+    kms, kn = tree.knn_timing(False)
+    plain_kernel_ms = kms / max(kn, 1)
     total_ms = e0.elapsed_time(e1)
     ms_step = total_ms / args.steps
     qps = N / (ms_step / 1e3)
@@ -162,20 +165,23 @@ def main():
         model = learned.fit(ttree, args.train_pairs, k=K, epochs=args.epochs, quantile=args.quantile, log=log)
         ttree.free()
         del tpts
-        grid = learned.occupancy(tree)
+        table = learned.feature_table(learned.occupancy(tree))
         train_s = time.time() - t0
         s1 = torch.zeros(6, dtype=torch.int64, device=dev)
-        learned.seeded_knn(tree, model, P[:1 << 20], K, grid=grid, stats=s1)
+        learned.seeded_knn(tree, model, P[:1 << 20], K, table=table, stats=s1)
         s1 = s1.cpu().numpy()
         for _ in range(args.warmup):
-            d, i = learned.seeded_knn(tree, model, P, K, grid=grid)
-        del d, i
+            d, i = learned.seeded_knn(tree, model, P, K, table=table)
+            del d, i
         torch.cuda.synchronize()
+        tree.knn_timing(True)
         e0.record()
         for _ in range(args.steps):
-            d, i = learned.seeded_knn(tree, model, P, K, grid=grid)
+            d, i = learned.seeded_knn(tree, model, P, K, table=table)
         e1.record()
         torch.cuda.synchronize()
+        kms, kn = tree.knn_timing(False)
+        seeded_kernel_ms = kms / max(kn, 1)
         seeded_ms = e0.elapsed_time(e1) / args.steps
         exact = bool(torch.equal(d, d_plain))
         learned_info = dict(seeded_qps=N / (seeded_ms / 1e3), seeded_ms_per_step=seeded_ms,
@@ -184,20 +190,28 @@ def main():
                             nodes_visited_per_query=float(s1[0] / max(s1[3], 1)),
                             points_tested_per_query=float(s1[1] / max(s1[3], 1)),
                             insertions_per_query=float(s1[4] / max(s1[3], 1)), margin=model.margin,
+                            traversal_kernel_ms=seeded_kernel_ms,
                             train_pairs=args.train_pairs, train_seconds=train_s)
         log(f"learned seed: {seeded_ms:.2f} ms/step ({N / seeded_ms / 1e3:.1f} Mq/s), "
             f"x{plain_ms / seeded_ms:.2f} vs plain, exact={exact}, fallback {learned_info['fallback_rate']:.4f}")
         del d, i
     del d_plain
+    kernel_ms = plain_kernel_ms
     if learned_info and learned_info["distances_identical"]:
         qps, ms_step = learned_info["seeded_qps"], learned_info["seeded_ms_per_step"]
-    achieved = bq * N / (ms_step / 1e3) / 1e9
+        kernel_ms = learned_info["traversal_kernel_ms"]
+    # roofline of the dominant kernel: algorithmic bytes of one launch (all N
+    # queries) over its average launch duration
+    achieved = bq * N / (kernel_ms / 1e3) / 1e9
+    sd = 32 if info.depth <= 32 else 64
+    kernel_name = f"k_knn_s<float,3,periodic,{K},{sd}>" if info.depth <= 64 else f"k_knn_w<float,3,periodic,{K}>"
 
     extra = dict(build_ms=build_ms, build_mpts_s=N / build_ms / 1e3, tree_nodes=int(info.nnodes),
                  tree_depth=int(info.depth), nodes_visited_per_query=float(st[0] / max(st[3], 1)),
                  points_tested_per_query=float(st[1] / max(st[3], 1)),
                  insertions_per_query=float(st[4] / max(st[3], 1)),
-                 plain_qps=plain_qps, plain_ms_per_step=plain_ms, learned=learned_info,
+                 plain_qps=plain_qps, plain_ms_per_step=plain_ms, plain_traversal_kernel_ms=plain_kernel_ms,
+                 learned=learned_info,
                  build_bytes_per_point=build_bytes_per_point(3, 4, s_i, N, 16))
     extra["build_roofline_frac"] = extra["build_bytes_per_point"] * N / (build_ms / 1e3) / 1e9 / HBM_PEAK_GBS
 
@@ -282,7 +296,7 @@ def main():
                    "path": "learned-seed" if learned_info and learned_info["distances_identical"] else "plain"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_knn<float,3,periodic,32>", "bytes_per_query": bq},
+                     "kernel": kernel_name, "kernel_ms": kernel_ms, "bytes_per_query": bq},
         "cpu_baseline": cpu,
         "extra": extra,
     }

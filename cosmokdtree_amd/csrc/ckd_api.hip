@@ -9,6 +9,9 @@
 namespace ckd {
 int query_knn(const ckd_tree *t, const void *q, int64_t nq, int k, int sorted, const float *seed, void *dist,
               int64_t *idx, int64_t *stats, hipStream_t s);
+int query_knn_learned(const ckd_tree *t, const void *q, int64_t nq, int k, int sorted, const float *params,
+                      float margin, int G, const uint16_t *table, void *dist, int64_t *idx, int64_t *stats,
+                      hipStream_t s);
 int query_ball(const ckd_tree *t, const void *q, int64_t nq, double r, int64_t *counts, const int64_t *off,
                void *dist, int64_t *idx, hipStream_t s);
 int query_box(const ckd_tree *t, const void *b, int64_t nb, int64_t *counts, const int64_t *off, int64_t *idx,
@@ -48,21 +51,23 @@ static int grow(void **buf, size_t *have, size_t bytes, void **p, hipStream_t s)
     return CKD_OK;
 }
 
-int workspace(ckd_tree *t, size_t bytes, void **p, hipStream_t s) { return grow(&t->ws, &t->ws_bytes, bytes, p, s); }
-
-int workspace2(ckd_tree *t, size_t bytes, void **p, hipStream_t s) {
-    return grow(&t->ws2, &t->ws2_bytes, bytes, p, s);
+int workspace(ckd_tree *t, int slot, size_t bytes, void **p, hipStream_t s) {
+    return grow(&t->ws[slot], &t->ws_bytes[slot], bytes, p, s);
 }
 
 static void release(ckd_tree *t) {
+    for (auto &ev : t->knn_events) {
+        (void)hipEventDestroy(ev.first);
+        (void)hipEventDestroy(ev.second);
+    }
+    t->knn_events.clear();
     if (t->set_off) (void)hipFree(t->set_off);
     t->set_off = nullptr;
-    if (t->ws) (void)hipFree(t->ws);
-    t->ws = nullptr;
-    t->ws_bytes = 0;
-    if (t->ws2) (void)hipFree(t->ws2);
-    t->ws2 = nullptr;
-    t->ws2_bytes = 0;
+    for (int i = 0; i < ckd_tree::NWS; i++) {
+        if (t->ws[i]) (void)hipFree(t->ws[i]);
+        t->ws[i] = nullptr;
+        t->ws_bytes[i] = 0;
+    }
     if (t->nodes) (void)hipFree(t->nodes);
     if (t->bbox) (void)hipFree(t->bbox);
     if (t->dbox) (void)hipFree(t->dbox);
@@ -258,6 +263,35 @@ int ckd_knn(const ckd_tree *t, const void *queries, int64_t nq, int32_t k, int32
     return query_knn(t, queries, nq, k, sorted, seed_r, dist, idx, stats, (hipStream_t)stream);
 }
 
+int ckd_knn_learned(const ckd_tree *t, const void *queries, int64_t nq, int32_t k, int32_t sorted,
+                    const float *params, float margin, int32_t G, const uint16_t *table, void *dist, int64_t *idx,
+                    int64_t *stats, void *stream) {
+    g_err[0] = 0;
+    if (!t || !params || !table) {
+        set_error("tree, params and table must not be NULL");
+        return CKD_ERR_ARG;
+    }
+    if (t->ndim != 3 || !t->periodic) {
+        set_error("learned seeds need a periodic 3-D tree");
+        return CKD_ERR_ARG;
+    }
+    if (k < 1 || k > 64) {
+        set_error("learned seeds need 1 <= k <= 64, got %d", k);
+        return CKD_ERR_ARG;
+    }
+    if (!(margin > 0.0f) || G < 1 || G > 1024) {
+        set_error("invalid margin or grid size");
+        return CKD_ERR_ARG;
+    }
+    if (nq < 0 || (nq > 0 && (!queries || !dist || !idx))) {
+        set_error("invalid query or output arrays");
+        return CKD_ERR_ARG;
+    }
+    if (nq == 0) return CKD_OK;
+    return query_knn_learned(t, queries, nq, k, sorted, params, margin, G, table, dist, idx, stats,
+                             (hipStream_t)stream);
+}
+
 int ckd_knn_config(ckd_tree *t, int32_t kernel, int32_t stack_depth) {
     g_err[0] = 0;
     if (!t) {
@@ -275,6 +309,32 @@ int ckd_knn_config(ckd_tree *t, int32_t kernel, int32_t stack_depth) {
     t->knn_kernel = kernel;
     t->knn_stack = stack_depth;
     return CKD_OK;
+}
+
+int ckd_knn_timing(ckd_tree *t, int32_t on, double *total_ms, int64_t *launches) {
+    g_err[0] = 0;
+    if (!t) {
+        set_error("tree must not be NULL");
+        return CKD_ERR_ARG;
+    }
+    double sum = 0;
+    int64_t cnt = 0;
+    int rc = CKD_OK;
+    for (auto &ev : t->knn_events) {
+        float ms = 0;
+        hipError_t e = hipEventSynchronize(ev.second);
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev.first, ev.second);
+        if (e != hipSuccess && rc == CKD_OK) rc = hip_fail(e, "hipEventElapsedTime", __FILE__, __LINE__);
+        sum += ms;
+        cnt++;
+        (void)hipEventDestroy(ev.first);
+        (void)hipEventDestroy(ev.second);
+    }
+    t->knn_events.clear();
+    if (total_ms) *total_ms = sum;
+    if (launches) *launches = cnt;
+    t->timing = on != 0;
+    return rc;
 }
 
 int ckd_ball_count(const ckd_tree *t, const void *queries, int64_t nq, double radius, int64_t *offsets,

@@ -29,6 +29,8 @@
 
 #include <cfloat>
 #include <string>
+#include <utility>
+#include <vector>
 
 #include "../../include/cosmokdtree_amd.h"
 
@@ -238,10 +240,13 @@ struct ckd_tree {
     int2 *wroot = nullptr;   // device [nsets] root pairs
     int knn_kernel = -1;     // kNN kernel choice, -1 = automatic (ckd_knn_config)
     int knn_stack = 0;       // stack depth of the stack kernel, 0 = automatic
-    void *ws = nullptr;      // grow-only query workspace (calls on one tree must not overlap)
-    size_t ws_bytes = 0;
-    void *ws2 = nullptr;     // second workspace (retry lists)
-    size_t ws2_bytes = 0;
+    int timing = 0;          // record events around kNN traversal launches (ckd_knn_timing)
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> knn_events;
+    // grow-only query workspaces (calls on one tree must not overlap):
+    // slot 0 query ordering, 1 seed retry, 2 learned features and seeds
+    static constexpr int NWS = 3;
+    void *ws[NWS] = {nullptr, nullptr, nullptr};
+    size_t ws_bytes[NWS] = {0, 0, 0};
 };
 
 namespace ckd {
@@ -261,8 +266,7 @@ TreeView<T> view(const ckd_tree *t) {
     return v;
 }
 int build_tree(ckd_tree *t, const void *points, hipStream_t s);
-// Device workspace of at least `bytes` owned by the tree (grown on demand;
-// growing synchronises `s` before releasing the old buffer).
-int workspace(ckd_tree *t, size_t bytes, void **p, hipStream_t s);
-int workspace2(ckd_tree *t, size_t bytes, void **p, hipStream_t s);
+// Device workspace `slot` of at least `bytes` owned by the tree (grown on
+// demand; growing synchronises `s` before releasing the old buffer).
+int workspace(ckd_tree *t, int slot, size_t bytes, void **p, hipStream_t s);
 }  // namespace ckd

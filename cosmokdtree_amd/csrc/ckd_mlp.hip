@@ -1,5 +1,6 @@
 // ckd_mlp.hip -- the learned search-radius proposal: features, a 32-64-64-1
-// GELU network on bf16 MFMA (v_mfma_f32_16x16x32_bf16, fp32 accumulation),
+// GELU network (sigmoid form) on bf16 MFMA (v_mfma_f32_16x16x32_bf16, fp32
+// accumulation),
 // its backward pass and Adam.  No reference counterpart: the network only
 // seeds the kNN pruning bound; a query that finds fewer than k points inside
 // the seed falls back to the plain search (ckd_query_impl.h), so results never
@@ -14,7 +15,7 @@
 // W2[64][64] (row = input unit), b2[64], w3[64], b3.
 #include <cmath>
 
-#include "ckd_internal.h"
+#include "ckd_traverse.h"
 
 namespace ckd {
 
@@ -37,9 +38,15 @@ __device__ __forceinline__ uint16_t f2bf(float x) {  // round to nearest even
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-__device__ __forceinline__ float gelu(float z) { return z * 0.5f * (1.0f + erff(z * 0.70710678118654752f)); }
+// GELU in its sigmoid form, z * sigmoid(1.702 z) (Hendrycks & Gimpel,
+// "Gaussian Error Linear Units", 2016): one exp and one reciprocal
+// instead of erf.
+constexpr float GELU_A = 1.702f;
+__device__ __forceinline__ float sigm(float x) { return __builtin_amdgcn_rcpf(1.0f + __expf(-x)); }
+__device__ __forceinline__ float gelu(float z) { return z * sigm(GELU_A * z); }
 __device__ __forceinline__ float gelu_grad(float z) {
-    return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * expf(-0.5f * z * z) * 0.3989422804014327f;
+    const float sg = sigm(GELU_A * z);
+    return sg + GELU_A * z * sg * (1.0f - sg);
 }
 
 // 8 consecutive bf16 (16 bytes) from LDS as an MFMA operand
@@ -379,36 +386,68 @@ __global__ void k_occupancy(const T *pts, int64_t n, int ps, double L0, double L
     atomicAdd(&grid[((int64_t)c[2] * G + c[1]) * G + c[0]], 1);
 }
 
-// 30 features + 2 zero pads per query, bf16.
+// bf16 log1p of every cell count (the 27 neighbourhood features are
+// lookups in it).
+__global__ void k_logtab(const int32_t *grid, int64_t cells, uint16_t *tab) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < cells) tab[i] = f2bf((float)log1p((double)grid[i]));
+}
+
+// (1/3) ln(count / volume) of a node's tight point box, each extent at
+// least 1e-6 L: the log inverse length scale of the local point density.
 template <typename T>
-__global__ void k_features(const T *q, int64_t nq, double L0, double L1, double L2, int G, const int32_t *grid,
-                           uint16_t *X) {
+__device__ __forceinline__ float density_feature(const TreeView<T> &tv, int32_t v, const double *Ls) {
+    const T *b = tv.dbox + (int64_t)v * 6;
+    double V = 1.0;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const double e = (double)b[3 + d] - (double)b[d], m = 1e-6 * Ls[d];
+        V = V * (e > m ? e : m);
+    }
+    return (float)(log((double)tv.range[v].count / V) / 3.0);
+}
+
+// 32 bf16 features per query: 3 positions (q + L/2) / L, the 27 log1p
+// counts of the query's 3x3x3 periodic cell neighbourhood (dz-major), and
+// the density feature of the smallest ancestor of the query's leaf (leaf
+// included) holding at least 16 points, then at least 64 points.
+template <typename T>
+__global__ void k_features(TreeView<T> tv, const T *q, int64_t nq, double L0, double L1, double L2, int G,
+                           const uint16_t *tab, const int32_t *leaf_of, uint16_t *X) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nq) return;
     const double Ls[3] = {L0, L1, L2};
     uint16_t f[NIN];
     int c[3];
+    T qq[3];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-        const double x = (double)q[i * 3 + d];
-        const double u = (x + Ls[d] / 2) / Ls[d];
+        qq[d] = q[i * 3 + d];
+        const double u = ((double)qq[d] + Ls[d] / 2) / Ls[d];
         f[d] = f2bf((float)u);
         int v = (int)floor(u * G);
         c[d] = v < 0 ? 0 : (v > G - 1 ? G - 1 : v);
     }
     int j = 3;
+#pragma unroll
     for (int dz = -1; dz <= 1; dz++)
+#pragma unroll
         for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
             for (int dx = -1; dx <= 1; dx++) {
                 const int x = (c[0] + dx + G) % G, y = (c[1] + dy + G) % G, z = (c[2] + dz + G) % G;
-                f[j++] = f2bf((float)log1p((double)grid[((int64_t)z * G + y) * G + x]));
+                f[j++] = tab[((int64_t)z * G + y) * G + x];
             }
-    f[30] = f[31] = 0;
+    int32_t v = leaf_of ? leaf_of[i] : find_leaf<T, 3>(tv, qq, 0);
+    while (tv.range[v].count < 16 && tv.nodes[v].parent >= 0) v = tv.nodes[v].parent;
+    f[30] = f2bf(density_feature<T>(tv, v, Ls));
+    while (tv.range[v].count < 64 && tv.nodes[v].parent >= 0) v = tv.nodes[v].parent;
+    f[31] = f2bf(density_feature<T>(tv, v, Ls));
     uint4 *dst = (uint4 *)(X + i * NIN);
 #pragma unroll
-    for (int v = 0; v < 4; v++)
-        dst[v] = make_uint4(f[8 * v] | ((uint32_t)f[8 * v + 1] << 16), f[8 * v + 2] | ((uint32_t)f[8 * v + 3] << 16),
-                            f[8 * v + 4] | ((uint32_t)f[8 * v + 5] << 16), f[8 * v + 6] | ((uint32_t)f[8 * v + 7] << 16));
+    for (int w = 0; w < 4; w++)
+        dst[w] = make_uint4(f[8 * w] | ((uint32_t)f[8 * w + 1] << 16), f[8 * w + 2] | ((uint32_t)f[8 * w + 3] << 16),
+                            f[8 * w + 4] | ((uint32_t)f[8 * w + 5] << 16), f[8 * w + 6] | ((uint32_t)f[8 * w + 7] << 16));
 }
 
 unsigned grid_blocks(int64_t rows) {
@@ -443,20 +482,33 @@ int ckd_occupancy(const ckd_tree *t, int32_t G, int32_t *grid, void *stream) {
     return CKD_OK;
 }
 
-int ckd_mlp_features(const void *queries, int64_t nq, int32_t prec, const double *boxsize, int32_t G,
-                     const int32_t *grid, uint16_t *X, void *stream) {
-    if (nq < 0 || !boxsize || !grid || (nq > 0 && (!queries || !X)) || (prec != 4 && prec != 8)) {
-        set_error("invalid ckd_mlp_features arguments");
+int ckd_mlp_table(const int32_t *grid, int32_t G, uint16_t *table, void *stream) {
+    if (!grid || !table || G < 1 || G > 1024) {
+        set_error("invalid ckd_mlp_table arguments");
+        return CKD_ERR_ARG;
+    }
+    const int64_t cells = (int64_t)G * G * G;
+    k_logtab<<<(unsigned)ceil_div(cells, 256), 256, 0, (hipStream_t)stream>>>(grid, cells, table);
+    CKD_HIP(hipGetLastError());
+    return CKD_OK;
+}
+
+int ckd_mlp_features(const ckd_tree *t, const void *queries, int64_t nq, int32_t G, const uint16_t *table,
+                     const int32_t *leaf_of, uint16_t *X, void *stream) {
+    if (!t || t->ndim != 3 || !t->periodic || nq < 0 || !table || G < 1 || G > 1024 ||
+        (nq > 0 && (!queries || !X))) {
+        set_error("invalid ckd_mlp_features arguments (a periodic 3-D tree is required)");
         return CKD_ERR_ARG;
     }
     if (nq == 0) return CKD_OK;
     hipStream_t s = (hipStream_t)stream;
     const unsigned g = (unsigned)ceil_div(nq, 256);
-    if (prec == 4)
-        k_features<float><<<g, 256, 0, s>>>((const float *)queries, nq, boxsize[0], boxsize[1], boxsize[2], G, grid, X);
+    if (t->prec == 4)
+        k_features<float><<<g, 256, 0, s>>>(view<float>(t), (const float *)queries, nq, t->L[0], t->L[1], t->L[2], G,
+                                            table, leaf_of, X);
     else
-        k_features<double><<<g, 256, 0, s>>>((const double *)queries, nq, boxsize[0], boxsize[1], boxsize[2], G, grid,
-                                             X);
+        k_features<double><<<g, 256, 0, s>>>(view<double>(t), (const double *)queries, nq, t->L[0], t->L[1], t->L[2],
+                                             G, table, leaf_of, X);
     CKD_HIP(hipGetLastError());
     return CKD_OK;
 }

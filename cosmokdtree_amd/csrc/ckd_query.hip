@@ -7,6 +7,10 @@ int query_knn_f32(const ckd_tree *, const void *, int64_t, int, int, const float
                   hipStream_t);
 int query_knn_f64(const ckd_tree *, const void *, int64_t, int, int, const float *, void *, int64_t *, int64_t *,
                   hipStream_t);
+int query_knn_learned_f32(const ckd_tree *, const void *, int64_t, int, int, const float *, float, int,
+                          const uint16_t *, void *, int64_t *, int64_t *, hipStream_t);
+int query_knn_learned_f64(const ckd_tree *, const void *, int64_t, int, int, const float *, float, int,
+                          const uint16_t *, void *, int64_t *, int64_t *, hipStream_t);
 int query_ball_f32(const ckd_tree *, const void *, int64_t, double, int64_t *, const int64_t *, void *, int64_t *,
                    hipStream_t);
 int query_ball_f64(const ckd_tree *, const void *, int64_t, double, int64_t *, const int64_t *, void *, int64_t *,
@@ -18,6 +22,13 @@ int query_knn(const ckd_tree *t, const void *q, int64_t nq, int k, int sorted, c
               int64_t *idx, int64_t *stats, hipStream_t s) {
     return t->prec == 4 ? query_knn_f32(t, q, nq, k, sorted, seed, dist, idx, stats, s)
                         : query_knn_f64(t, q, nq, k, sorted, seed, dist, idx, stats, s);
+}
+
+int query_knn_learned(const ckd_tree *t, const void *q, int64_t nq, int k, int sorted, const float *params,
+                      float margin, int G, const uint16_t *table, void *dist, int64_t *idx, int64_t *stats,
+                      hipStream_t s) {
+    return t->prec == 4 ? query_knn_learned_f32(t, q, nq, k, sorted, params, margin, G, table, dist, idx, stats, s)
+                        : query_knn_learned_f64(t, q, nq, k, sorted, params, margin, G, table, dist, idx, stats, s);
 }
 
 int query_ball(const ckd_tree *t, const void *q, int64_t nq, double r, int64_t *counts, const int64_t *off,

@@ -5,7 +5,14 @@ namespace ckd {
 
 int query_knn_f64(const ckd_tree *t, const void *q, int64_t nq, int k, int sorted, const float *seed, void *dist,
                   int64_t *idx, int64_t *stats, hipStream_t s) {
-    return knn_t<double>(t, q, nq, k, sorted, seed, dist, idx, stats, s);
+    return knn_t<double>(t, q, nq, k, sorted, seed, nullptr, dist, idx, stats, s);
+}
+
+int query_knn_learned_f64(const ckd_tree *t, const void *q, int64_t nq, int k, int sorted, const float *params,
+                          float margin, int G, const uint16_t *table, void *dist, int64_t *idx, int64_t *stats,
+                          hipStream_t s) {
+    const Learned ln{params, margin, G, table};
+    return knn_t<double>(t, q, nq, k, sorted, nullptr, &ln, dist, idx, stats, s);
 }
 
 int query_ball_f64(const ckd_tree *t, const void *q, int64_t nq, double r, int64_t *counts, const int64_t *off,

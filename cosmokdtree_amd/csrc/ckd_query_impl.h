@@ -69,14 +69,14 @@ __device__ __forceinline__ void flush_stats(Counters c, int64_t nqw, int64_t *st
 }
 
 // One traversal per query (knn_one).  With nq_dev the query count is read
-// on the device (the retry list of the wave kernels; those queries are
-// not counted again in stats[3]).
+// on the device (the last retry round of the wave kernels; those queries
+// are not counted again in stats[3]).
 template <typename T, int D, bool PER, int KMAX>
-__global__ __launch_bounds__(QT) void k_knn(TreeView<T> tv, const T *queries, int64_t nq, const int32_t *nq_dev,
+__global__ __launch_bounds__(QT) void k_knn(TreeView<T> tv, const T *queries, int64_t nq, const int64_t *nq_dev,
                                             int k, int sorted, const float *seed, const int32_t *order,
                                             const int32_t *leaf_of, T *dout, int64_t *iout, int64_t *stats) {
     const int64_t t = (int64_t)blockIdx.x * QT + threadIdx.x;
-    const int64_t n = nq_dev ? (int64_t)*nq_dev : nq;
+    const int64_t n = nq_dev ? *nq_dev : nq;
     Counters c;
     if (t < n) {
         const int64_t qi = order ? (int64_t)order[t] : t;
@@ -94,32 +94,59 @@ __global__ __launch_bounds__(QT) void k_knn(TreeView<T> tv, const T *queries, in
     if (stats) flush_stats(c, (t < n && !nq_dev) ? 1 : 0, stats);
 }
 
+// Where the wave kernels send a seeded query whose seed held fewer than k
+// points: with `flag`, round one marks its position (flag[t] = 1, compacted
+// in leaf order afterwards) and proposes a larger seed from the m points it
+// found, s' = 1.25 s ((k + 1) / (m + 1))^(1/3), the radius that would hold
+// k + 1 points at the density seen inside s; otherwise (round two) the query
+// index is appended to `list` for the unseeded per-query kernel.
+struct Retry {
+    int64_t *flag = nullptr;
+    float *seed_next = nullptr;
+    int32_t *list = nullptr;
+    int64_t *count = nullptr;
+};
+
+__device__ __forceinline__ void send_retry(const Retry &rt, int64_t t, int64_t qi, float seed, int found, int k) {
+    if (rt.flag) {
+        rt.flag[t] = 1;
+        rt.seed_next[qi] = 1.25f * seed * cbrtf((float)(k + 1) / (float)(found + 1));
+    } else {
+        rt.list[atomicAdd((unsigned long long *)rt.count, 1ull)] = (int32_t)qi;
+    }
+}
+
 // Wave-uniform variant (knn_wave): queries in leaf order, one traversal per
-// wave, parent-pointer walk (any tree depth).
+// wave, parent-pointer walk (any tree depth).  nq_dev as for k_knn.
 template <typename T, int D, bool PER, int KMAX>
-__global__ __launch_bounds__(QT, 3) void k_knn_w(TreeView<T> tv, const T *queries, int64_t nq, int k, int sorted,
-                                              const float *seed, const int32_t *order, T *dout, int64_t *iout,
-                                              int64_t *stats, int32_t *retry, int32_t *nretry) {
+__global__ __launch_bounds__(QT, 3) void k_knn_w(TreeView<T> tv, const T *queries, int64_t nq,
+                                              const int64_t *nq_dev, int k, int sorted, const float *seed,
+                                              const int32_t *order, T *dout, int64_t *iout, int64_t *stats,
+                                              Retry rt) {
     __shared__ T buf_d[CBUF * QT];
     __shared__ int32_t buf_s[CBUF * QT];
     const int64_t t = (int64_t)blockIdx.x * QT + threadIdx.x;
+    const int64_t n = nq_dev ? *nq_dev : nq;
+    if ((int64_t)blockIdx.x * QT + (threadIdx.x & ~(WAVE - 1)) >= n) return;  // whole wave idle
     Counters c;
-    const bool active = t < nq;
+    const bool active = t < n;
     const int64_t qi = active ? (order ? (int64_t)order[t] : t) : 0;
     T q[D];
 #pragma unroll
     for (int d = 0; d < D; d++) q[d] = active ? queries[qi * D + d] : T(0);
     T seed2 = T(FLT_MAX);
+    float sr = 0.0f;
     if (seed && active) {
-        const T sr = (T)seed[qi];
-        seed2 = sr * sr;
+        sr = seed[qi];
+        seed2 = (T)sr * (T)sr;
     }
+    int found = 0;
     const bool done = knn_wave<T, D, PER, KMAX>(tv, q, active, 0, k, sorted, seed2, dout + qi * k, iout + qi * k,
-                                                0, c, buf_d, buf_s, threadIdx.x, QT);
-    if (!done) retry[atomicAdd(nretry, 1)] = (int32_t)qi;
+                                                0, c, buf_d, buf_s, threadIdx.x, QT, found);
+    if (!done) send_retry(rt, t, qi, sr, found, k);
     if (stats) {
         if (!active) c = Counters();
-        flush_stats(c, active ? 1 : 0, stats);
+        flush_stats(c, active && !nq_dev ? 1 : 0, stats);
     }
 }
 
@@ -128,33 +155,45 @@ __global__ __launch_bounds__(QT, 3) void k_knn_w(TreeView<T> tv, const T *querie
 // cheaper than the LDS round trip and the occupancy the buffer costs); 8
 // slots per lane in double precision keep the LDS within 4 blocks per CU.
 template <typename T, int D, bool PER, int KMAX, int SD>
-__global__ __launch_bounds__(QT, 3) void k_knn_s(TreeView<T> tv, int2 root, const T *queries, int64_t nq, int k,
-                                              int sorted, const float *seed, const int32_t *order, T *dout,
-                                              int64_t *iout, int64_t *stats, int32_t *retry, int32_t *nretry) {
+__global__ __launch_bounds__(QT, 3) void k_knn_s(TreeView<T> tv, int2 root, const T *queries, int64_t nq,
+                                              const int64_t *nq_dev, int k, int sorted, const float *seed,
+                                              const int32_t *order, T *dout, int64_t *iout, int64_t *stats,
+                                              Retry rt) {
     constexpr int CB = KMAX < 16 ? 0 : (sizeof(T) == 4 ? 16 : 8);
     __shared__ T buf_d[CB > 0 ? CB * QT : 1];
     __shared__ int32_t buf_s[CB > 0 ? CB * QT : 1];
     __shared__ StackEntry<T, D> stk[QT / WAVE][SD];
     const int64_t t = (int64_t)blockIdx.x * QT + threadIdx.x;
+    const int64_t n = nq_dev ? *nq_dev : nq;
+    if ((int64_t)blockIdx.x * QT + (threadIdx.x & ~(WAVE - 1)) >= n) return;  // whole wave idle
     Counters c;
-    const bool active = t < nq;
+    const bool active = t < n;
     const int64_t qi = active ? (order ? (int64_t)order[t] : t) : 0;
     T q[D];
 #pragma unroll
     for (int d = 0; d < D; d++) q[d] = active ? queries[qi * D + d] : T(0);
     T seed2 = T(FLT_MAX);
+    float sr = 0.0f;
     if (seed && active) {
-        const T sr = (T)seed[qi];
-        seed2 = sr * sr;
+        sr = seed[qi];
+        seed2 = (T)sr * (T)sr;
     }
+    int found = 0;
     const bool done = knn_stack<T, D, PER, KMAX, SD, CB>(tv, root, q, active, k, sorted, seed2, dout + qi * k,
                                                           iout + qi * k, 0, c, buf_d, buf_s, threadIdx.x, QT,
-                                                          stk[threadIdx.x / WAVE]);
-    if (!done) retry[atomicAdd(nretry, 1)] = (int32_t)qi;
+                                                          stk[threadIdx.x / WAVE], found);
+    if (!done) send_retry(rt, t, qi, sr, found, k);
     if (stats) {
         if (!active) c = Counters();
-        flush_stats(c, active ? 1 : 0, stats);
+        flush_stats(c, active && !nq_dev ? 1 : 0, stats);
     }
+}
+
+// Positions flagged by round one, compacted in order: list[off[t]] = query.
+__global__ void k_compact_retry(const int64_t *flag, const int64_t *off, int64_t n, const int32_t *order,
+                                int32_t *list) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n && flag[t]) list[off[t]] = order ? order[t] : (int32_t)t;
 }
 
 // kNN for k beyond the register list: the max-heap of the reference
@@ -292,57 +331,120 @@ bool stack_ok(const ckd_tree *t, int sd) {
     return D <= 3 && t->wide != nullptr && t->depth <= sd;
 }
 
+// What ckd_knn_learned adds: the network that proposes the seeds.
+struct Learned {
+    const float *params;
+    float margin;
+    int G;
+    const uint16_t *table;
+};
+
 template <typename T, int D, bool PER, int KMAX>
-void launch_lane(const TreeView<T> &tv, unsigned g, const T *q, int64_t nq, const int32_t *nq_dev, int k,
-                 int sorted, const float *seed, const int32_t *order, const int32_t *leaf_of, T *dist, int64_t *idx,
-                 int64_t *stats, hipStream_t s) {
-    k_knn<T, D, PER, KMAX><<<g, QT, 0, s>>>(tv, q, nq, nq_dev, k, sorted, seed, order, leaf_of, dist, idx, stats);
+int launch_wave(const ckd_tree *t, const TreeView<T> &tv, int kind, int sd, unsigned g, const T *q, int64_t nq,
+                const int64_t *nq_dev, int k, int sorted, const float *seed, const int32_t *order, T *dist,
+                int64_t *idx, int64_t *stats, const Retry &rt, hipStream_t s) {
+    if (kind == CKD_KNN_STACK) {
+        if constexpr (D <= 3) {
+            const int2 root = make_int2(0, t->nnodes > 1 ? -1 : (int32_t)t->n);
+            if (sd <= 32)
+                k_knn_s<T, D, PER, KMAX, 32><<<g, QT, 0, s>>>(tv, root, q, nq, nq_dev, k, sorted, seed, order, dist,
+                                                               idx, stats, rt);
+            else
+                k_knn_s<T, D, PER, KMAX, 64><<<g, QT, 0, s>>>(tv, root, q, nq, nq_dev, k, sorted, seed, order, dist,
+                                                               idx, stats, rt);
+        }
+    } else if constexpr (D <= 3) {
+        k_knn_w<T, D, PER, KMAX><<<g, QT, 0, s>>>(tv, q, nq, nq_dev, k, sorted, seed, order, dist, idx, stats, rt);
+    }
+    CKD_HIP(hipGetLastError());
+    return CKD_OK;
 }
+
+// Events around the main traversal launch, when the tree records them.
+struct LaunchTimer {
+    const ckd_tree *t;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int start(hipStream_t s) {
+        if (!t->timing) return CKD_OK;
+        CKD_HIP(hipEventCreate(&e0));
+        CKD_HIP(hipEventCreate(&e1));
+        CKD_HIP(hipEventRecord(e0, s));
+        return CKD_OK;
+    }
+    int stop(hipStream_t s) {
+        if (!t->timing) return CKD_OK;
+        CKD_HIP(hipEventRecord(e1, s));
+        const_cast<ckd_tree *>(t)->knn_events.emplace_back(e0, e1);
+        return CKD_OK;
+    }
+};
 
 template <typename T, int D, bool PER, int KMAX>
 int knn_launch_k(const ckd_tree *t, const TreeView<T> &tv, int kind, int sd, const T *q, int64_t nq, int k,
                  int sorted, const float *seed, const int32_t *order, const int32_t *leaf_of, T *dist, int64_t *idx,
                  int64_t *stats, hipStream_t s) {
     const unsigned g = (unsigned)ceil_div(nq, QT);
+    LaunchTimer tm{t};
+    int rc;
     if (kind == CKD_KNN_LANE) {
-        launch_lane<T, D, PER, KMAX>(tv, g, q, nq, nullptr, k, sorted, seed, order, leaf_of, dist, idx, stats, s);
+        if ((rc = tm.start(s))) return rc;
+        k_knn<T, D, PER, KMAX><<<g, QT, 0, s>>>(tv, q, nq, nullptr, k, sorted, seed, order, leaf_of, dist, idx, stats);
         CKD_HIP(hipGetLastError());
-        return CKD_OK;
+        return tm.stop(s);
     }
-    // wave kernels: seeded queries whose seed held fewer than k points are
-    // listed in `retry` and searched again by the per-query kernel
-    int32_t *retry = nullptr, *nretry = nullptr;
-    if (seed) {
-        void *ws2 = nullptr;
-        int rc = workspace2(const_cast<ckd_tree *>(t), sizeof(int32_t) * (size_t)(nq + 64), &ws2, s);
-        if (rc) return rc;
-        nretry = (int32_t *)ws2;
-        retry = nretry + 64;
-        k_fill_i32<<<1, 64, 0, s>>>(nretry, 1, 0);
+    if (!seed) {
+        if ((rc = tm.start(s))) return rc;
+        if ((rc = launch_wave<T, D, PER, KMAX>(t, tv, kind, sd, g, q, nq, nullptr, k, sorted, nullptr, order, dist, idx,
+                                               stats, Retry(), s)))
+            return rc;
+        return tm.stop(s);
     }
-    if (kind == CKD_KNN_STACK) {
-        if constexpr (D <= 3) {
-            const int2 root = make_int2(0, t->nnodes > 1 ? -1 : (int32_t)t->n);
-            if (sd <= 32)
-                k_knn_s<T, D, PER, KMAX, 32><<<g, QT, 0, s>>>(tv, root, q, nq, k, sorted, seed, order, dist, idx,
-                                                               stats, retry, nretry);
-            else
-                k_knn_s<T, D, PER, KMAX, 64><<<g, QT, 0, s>>>(tv, root, q, nq, k, sorted, seed, order, dist, idx,
-                                                               stats, retry, nretry);
-        }
-    } else if constexpr (D <= 3) {
-        k_knn_w<T, D, PER, KMAX><<<g, QT, 0, s>>>(tv, q, nq, k, sorted, seed, order, dist, idx, stats, retry, nretry);
-    }
-    CKD_HIP(hipGetLastError());
-    if (seed)
-        launch_lane<T, D, PER, KMAX>(tv, g, q, nq, nretry, k, sorted, nullptr, retry, nullptr, dist, idx, stats, s);
+    // Seeded: round one with the given seeds; round two, in leaf order, for
+    // the queries whose seed held fewer than k points, with larger seeds;
+    // round three, unseeded and one traversal per query, for what is left.
+    // workspace: flag, off (int64 x nq+1), list, list2 (int32 x nq),
+    // seed_next (float x nq), count2, scan scratch
+    const size_t nb = (size_t)nq;
+    const size_t bytes = 2 * 8 * (nb + 1) + 2 * 4 * nb + 4 * nb + 64 + scan_tmp_bytes(nq) + 8 * 256;  // 7 takes
+    void *ws = nullptr;
+    if ((rc = workspace(const_cast<ckd_tree *>(t), 1, bytes, &ws, s))) return rc;
+    char *p = (char *)ws;
+    auto take = [&](size_t b) {
+        char *r = p;
+        p += (b + 255) & ~(size_t)255;
+        return (void *)r;
+    };
+    int64_t *flag = (int64_t *)take(8 * (nb + 1)), *off = (int64_t *)take(8 * (nb + 1));
+    int32_t *list = (int32_t *)take(4 * nb), *list2 = (int32_t *)take(4 * nb);
+    float *seed_next = (float *)take(4 * nb);
+    int64_t *count2 = (int64_t *)take(64);
+    void *scr = take(scan_tmp_bytes(nq));
+    k_fill_i64<<<(unsigned)ceil_div(nq, 256), 256, 0, s>>>(flag, nq, 0);
+    k_fill_i64<<<1, 64, 0, s>>>(count2, 1, 0);
+    Retry r1;
+    r1.flag = flag;
+    r1.seed_next = seed_next;
+    if ((rc = tm.start(s))) return rc;
+    if ((rc = launch_wave<T, D, PER, KMAX>(t, tv, kind, sd, g, q, nq, nullptr, k, sorted, seed, order, dist, idx,
+                                           stats, r1, s)))
+        return rc;
+    if ((rc = tm.stop(s))) return rc;
+    if ((rc = scan_i64(flag, off, nq, true, scr, s))) return rc;
+    k_compact_retry<<<(unsigned)ceil_div(nq, 256), 256, 0, s>>>(flag, off, nq, order, list);
+    Retry r2;
+    r2.list = list2;
+    r2.count = count2;
+    if ((rc = launch_wave<T, D, PER, KMAX>(t, tv, kind, sd, g, q, nq, off + nq, k, sorted, seed_next, list, dist, idx,
+                                           stats, r2, s)))
+        return rc;
+    k_knn<T, D, PER, KMAX><<<g, QT, 0, s>>>(tv, q, nq, count2, k, sorted, nullptr, list2, nullptr, dist, idx, stats);
     CKD_HIP(hipGetLastError());
     return CKD_OK;
 }
 
 template <typename T, int D, bool PER>
-int knn_launch(const ckd_tree *t, const T *q, int64_t nq, int k, int sorted, const float *seed, T *dist,
-               int64_t *idx, int64_t *stats, hipStream_t s) {
+int knn_launch(const ckd_tree *t, const T *q, int64_t nq, int k, int sorted, const float *seed, const Learned *ln,
+               T *dist, int64_t *idx, int64_t *stats, hipStream_t s) {
     const TreeView<T> tv = view<T>(t);
     const unsigned g = (unsigned)ceil_div(nq, QT);
     if (k <= 64 && t->n < INT32_MAX) {
@@ -359,7 +461,7 @@ int knn_launch(const ckd_tree *t, const T *q, int64_t nq, int k, int sorted, con
             // offsets (int64 x nnodes+1), scan scratch
             const size_t a = 3 * sizeof(int32_t) * (size_t)nq, b = sizeof(int64_t) * (size_t)(t->nnodes + 1);
             void *ws = nullptr;
-            int rc = workspace(const_cast<ckd_tree *>(t), a + 2 * b + scan_tmp_bytes(t->nnodes) + 256, &ws, s);
+            int rc = workspace(const_cast<ckd_tree *>(t), 0, a + 2 * b + scan_tmp_bytes(t->nnodes) + 256, &ws, s);
             if (rc) return rc;
             int32_t *lo = (int32_t *)ws, *rk = lo + nq, *ord = rk + nq;
             int64_t *cnt = (int64_t *)((char *)ws + ((a + 255) & ~(size_t)255));
@@ -373,10 +475,26 @@ int knn_launch(const ckd_tree *t, const T *q, int64_t nq, int k, int sorted, con
             order = ord;
             leaf_of = lo;
         }
+        if (ln) {
+            // learned seeds: features (reusing the leaves just found) and the
+            // network, into workspace slot 2
+            void *ws = nullptr;
+            int rc = workspace(const_cast<ckd_tree *>(t), 2, (size_t)nq * (64 + 4) + 256, &ws, s);
+            if (rc) return rc;
+            uint16_t *X = (uint16_t *)ws;
+            float *sd_out = (float *)((char *)ws + (size_t)nq * 64);
+            if ((rc = ckd_mlp_features(t, q, nq, ln->G, ln->table, leaf_of, X, s))) return rc;
+            if ((rc = ckd_mlp_forward(ln->params, X, nq, ln->margin, 1, sd_out, s))) return rc;
+            seed = sd_out;
+        }
         if (k <= 8) return knn_launch_k<T, D, PER, 8>(t, tv, kind, sd, q, nq, k, sorted, seed, order, leaf_of, dist, idx, stats, s);
         if (k <= 16) return knn_launch_k<T, D, PER, 16>(t, tv, kind, sd, q, nq, k, sorted, seed, order, leaf_of, dist, idx, stats, s);
         if (k <= 32) return knn_launch_k<T, D, PER, 32>(t, tv, kind, sd, q, nq, k, sorted, seed, order, leaf_of, dist, idx, stats, s);
         return knn_launch_k<T, D, PER, 64>(t, tv, kind, sd, q, nq, k, sorted, seed, order, leaf_of, dist, idx, stats, s);
+    }
+    if (ln) {
+        set_error("learned seeds need k <= 64");
+        return CKD_ERR_ARG;
     }
     k_knn_heap<T, D, PER><<<g, QT, 0, s>>>(tv, q, nq, k, dist, idx);
     CKD_HIP(hipGetLastError());
@@ -425,12 +543,12 @@ int box_launch(const ckd_tree *t, const T *b, int64_t nb, int64_t *counts, const
     }
 
 template <typename T>
-int knn_t(const ckd_tree *t, const void *q, int64_t nq, int k, int sorted, const float *seed, void *dist,
-          int64_t *idx, int64_t *stats, hipStream_t s) {
+int knn_t(const ckd_tree *t, const void *q, int64_t nq, int k, int sorted, const float *seed, const Learned *ln,
+          void *dist, int64_t *idx, int64_t *stats, hipStream_t s) {
     if (t->periodic) {
-        CKD_DIM_SWITCH(t->ndim, (knn_launch<T, D_, true>(t, (const T *)q, nq, k, sorted, seed, (T *)dist, idx, stats, s)))
+        CKD_DIM_SWITCH(t->ndim, (knn_launch<T, D_, true>(t, (const T *)q, nq, k, sorted, seed, ln, (T *)dist, idx, stats, s)))
     } else {
-        CKD_DIM_SWITCH(t->ndim, (knn_launch<T, D_, false>(t, (const T *)q, nq, k, sorted, seed, (T *)dist, idx, stats, s)))
+        CKD_DIM_SWITCH(t->ndim, (knn_launch<T, D_, false>(t, (const T *)q, nq, k, sorted, seed, ln, (T *)dist, idx, stats, s)))
     }
     return CKD_ERR_ARG;
 }

@@ -222,7 +222,7 @@ __device__ __forceinline__ T wave_uniform(T v) {
 template <typename T, int D, bool PER, int KMAX>
 __device__ __forceinline__ bool knn_wave(const TreeView<T> &tv, const T (&q)[D], bool active, int32_t root, int k,
                                          int sorted, T seed2, T *drow, int64_t *irow, int64_t idx_base,
-                                         Counters &c, T *buf_d, int32_t *buf_s, int tid, int nthreads) {
+                                         Counters &c, T *buf_d, int32_t *buf_s, int tid, int nthreads, int &found) {
     T bd[KMAX];
     int32_t bs[KMAX];
     // near-first reference: mean of the active queries, made wave-uniform
@@ -323,6 +323,9 @@ __device__ __forceinline__ bool knn_wave(const TreeView<T> &tv, const T (&q)[D],
     if (!active) return true;
     if (bs[KMAX - 1] < 0 && init != T(FLT_MAX)) {
         c.reruns++;
+        found = 0;
+#pragma unroll
+        for (int j = 0; j < KMAX; j++) found += bs[j] >= 0 ? 1 : 0;
         return false;
     }
     bool fix = false;
@@ -593,12 +596,14 @@ __device__ __forceinline__ bool stack_walk(const TreeView<T> &tv, const T (&q)[D
 }
 
 // Returns false for a seeded lane whose seed held fewer than k points (it
-// writes nothing then; the caller searches it again unseeded).  `stk` is
-// this wave's stack of SD entries in LDS.
+// writes nothing then and sets `found` to the number it held; the caller
+// searches it again with a larger or no seed).  `stk` is this wave's stack
+// of SD entries in LDS.
 template <typename T, int D, bool PER, int KMAX, int SD, int CB>
 __device__ __forceinline__ bool knn_stack(const TreeView<T> &tv, int2 root, const T (&q)[D], bool active, int k,
                                           int sorted, T seed2, T *drow, int64_t *irow, int64_t idx_base, Counters &c,
-                                          T *buf_d, int32_t *buf_s, int tid, int nthreads, StackEntry<T, D> *stk) {
+                                          T *buf_d, int32_t *buf_s, int tid, int nthreads, StackEntry<T, D> *stk,
+                                          int &found) {
     T bd[KMAX];
     int32_t bs[KMAX];
     const T init = seed2 < T(FLT_MAX) ? seed2 : T(FLT_MAX);
@@ -623,6 +628,9 @@ __device__ __forceinline__ bool knn_stack(const TreeView<T> &tv, int2 root, cons
     if (!active) return true;
     if (bs[KMAX - 1] < 0 && init != T(FLT_MAX)) {
         c.reruns++;
+        found = 0;
+#pragma unroll
+        for (int j = 0; j < KMAX; j++) found += bs[j] >= 0 ? 1 : 0;
         return false;
     }
     knn_write_row<T, KMAX>(tv, bd, bs, k, sorted, drow, irow, idx_base);

@@ -100,6 +100,15 @@ class KDTree(int):
         check(lib.ckd_knn_config(self._ptr, _lib.KNN_KERNELS[kernel], int(stack)), "knn_config")
         return self
 
+    def knn_timing(self, on: bool = True):
+        """Start (on=True) or stop recording HIP events around this tree's kNN
+        traversal kernel launches; returns (summed ms, launches) of the
+        launches recorded since the previous call (synchronises them)."""
+        self._check_alive()
+        ms, n = C.c_double(0), C.c_int64(0)
+        check(lib.ckd_knn_timing(self._ptr, int(bool(on)), C.byref(ms), C.byref(n)), "knn_timing")
+        return ms.value, n.value
+
     # ------------------------------------------------------------ device API
     def _queries(self, q) -> torch.Tensor:
         q = torch.as_tensor(q, dtype=self.torch_dtype, device=self.device)

@@ -2,8 +2,10 @@
 
 A small network predicts, per query, the log of its k-th neighbour
 distance from where the query sits and how crowded its surroundings are:
-30 features = the query position scaled to [0, 1) per axis and log1p of the
-point counts in the 27 cells around it in a G^3 occupancy grid (G = 64).
+32 features = the query position scaled to [0, 1) per axis, log1p of the
+point counts in the 27 cells around it in a G^3 occupancy grid (G = 64),
+and two local densities read off the tree itself (the tight point boxes of
+the smallest ancestors of the query's leaf holding 16 and 64 points).
 The network (32 -> 64 -> 64 -> 1, GELU) runs on bf16 MFMA with fp32
 accumulation, and is trained on the GPU by the library's own forward,
 backward and Adam kernels (include/cosmokdtree_amd.h, ckd_mlp_*), on
@@ -33,8 +35,11 @@ from .kdtree import KDTree, _stream, _vp
 lib.ckd_mlp_nparam.restype = C.c_int
 lib.ckd_mlp_nparam.argtypes = []
 lib.ckd_occupancy.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
-lib.ckd_mlp_features.argtypes = [C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
+lib.ckd_mlp_table.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
+lib.ckd_mlp_features.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
                                  C.c_void_p]
+lib.ckd_knn_learned.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_void_p, C.c_float,
+                                C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
 lib.ckd_mlp_forward.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_float, C.c_int32, C.c_void_p, C.c_void_p]
 lib.ckd_mlp_grad.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]
 lib.ckd_mlp_adam.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_float, C.c_void_p]
@@ -50,7 +55,7 @@ def init_params(seed: int = 0) -> np.ndarray:
     p = np.zeros(NPARAM, dtype=np.float32)
     o = 0
     W1 = p[o:o + NIN * NH].reshape(NIN, NH)
-    W1[:30] = rng.standard_normal((30, NH)) * math.sqrt(2.0 / 30)
+    W1[:] = rng.standard_normal((NIN, NH)) * math.sqrt(2.0 / NIN)
     o += NIN * NH + NH
     p[o:o + NH * NH] = (rng.standard_normal((NH, NH)) * math.sqrt(2.0 / NH)).ravel()
     o += NH * NH + NH
@@ -64,13 +69,20 @@ def occupancy(tree: KDTree, G: int = GRID) -> torch.Tensor:
     return grid
 
 
-def features(queries: torch.Tensor, boxsize, grid: torch.Tensor, G: int = GRID) -> torch.Tensor:
-    """(nq, 32) bf16 features, stored as int16 bit patterns."""
-    q = queries.contiguous()
-    prec = 4 if q.dtype == torch.float32 else 8
+def feature_table(grid: torch.Tensor, G: int = GRID) -> torch.Tensor:
+    """bf16 log1p of every grid count (int16 bit patterns)."""
+    tab = torch.empty(G ** 3, dtype=torch.int16, device=grid.device)
+    check(lib.ckd_mlp_table(_vp(grid), G, _vp(tab), _stream()), "feature table")
+    return tab
+
+
+def features(tree: KDTree, queries: torch.Tensor, table: torch.Tensor, G: int = GRID, leaf_of=None) -> torch.Tensor:
+    """(nq, 32) bf16 features (int16 bit patterns): 3 positions, 27 grid
+    neighbourhood counts, 2 tree density features."""
+    q = tree._queries(queries)
     X = torch.empty((q.shape[0], NIN), dtype=torch.int16, device=q.device)
-    bs = (C.c_double * 3)(*[float(x) for x in boxsize])
-    check(lib.ckd_mlp_features(_vp(q), q.shape[0], prec, bs, G, _vp(grid), _vp(X), _stream()), "features")
+    check(lib.ckd_mlp_features(tree._ptr, _vp(q), q.shape[0], G, _vp(table), _vp(leaf_of), _vp(X), _stream()),
+          "features")
     return X
 
 
@@ -155,8 +167,7 @@ def training_pairs(tree: KDTree, queries: torch.Tensor, k: int, G: int = GRID):
     """(features, log r_k, r_k) for queries against `tree` (exact kNN)."""
     d, _ = tree.knn(queries, k, sorted=True)
     r = d[:, k - 1].float()
-    grid = occupancy(tree, G)
-    X = features(queries, tree.boxsize, grid, G)
+    X = features(tree, queries, feature_table(occupancy(tree, G), G), G)
     return X, torch.log(r), r
 
 
@@ -187,10 +198,20 @@ def train_tree_points(tree: KDTree) -> torch.Tensor:
 
 
 def seeded_knn(tree: KDTree, model: RadiusModel, queries: torch.Tensor, k: int, sorted: bool = True,
-               grid: torch.Tensor | None = None, stats=None):
-    """kNN with learned seeds: features -> network -> seeded traversal."""
-    if grid is None:
-        grid = occupancy(tree)
-    X = features(queries, tree.boxsize, grid)
-    seed = model.seeds(X)
-    return tree.knn(queries, k, sorted=sorted, seed_r=seed, stats=stats)
+               table: torch.Tensor | None = None, stats=None, margin: float | None = None):
+    """kNN with learned seeds in one library call (ckd_knn_learned): leaves,
+    features, network, seeded traversal and the re-search of short seeds.
+    `table` is feature_table(occupancy(tree)), computed once per tree."""
+    tree._check_alive()
+    if table is None:
+        table = feature_table(occupancy(tree))
+    q = tree._queries(queries)
+    nq = q.shape[0]
+    dist = torch.empty((nq, k), dtype=tree.torch_dtype, device=tree.device)
+    idx = torch.empty((nq, k), dtype=torch.int64, device=tree.device)
+    if nq == 0:
+        return dist, idx
+    m = model.margin if margin is None else margin
+    check(lib.ckd_knn_learned(tree._ptr, _vp(q), nq, int(k), int(bool(sorted)), _vp(model.params), float(m), GRID,
+                              _vp(table), _vp(dist), _vp(idx), _vp(stats), _stream()), "knn_learned")
+    return dist, idx

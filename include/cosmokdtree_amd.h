@@ -103,7 +103,7 @@ int ckd_get_info(const ckd_tree *tree, ckd_tree_info *info);
  * reference's distance() (cosmokdtree.f90:1441-1465).
  * seed_r [dev, nq floats] or NULL: a per-query search radius proposal; a
  * query whose seeded search finds fewer than k points is searched again
- * without it, so results never depend on the seed.
+ * (with a larger seed, then without), so results never depend on the seed.
  * stats [dev, 6 int64] or NULL: accumulates {nodes visited, leaf points
  * tested, queries re-run after a short seed, queries, candidates inserted
  * into a result list, parent steps climbed or deferred subtrees popped}.
@@ -127,6 +127,18 @@ int ckd_knn(const ckd_tree *tree, const void *queries, int64_t nq, int32_t k, in
  * results; only the speed differs.  Used by the autotuner.
  */
 int ckd_knn_config(ckd_tree *tree, int32_t kernel, int32_t stack_depth);
+
+/*
+ * Launch timing of the kNN traversal kernel of one tree (no reference
+ * counterpart; for benchmarks and roofline figures).  on != 0 records a
+ * pair of HIP events, on the call's stream, around every traversal kernel
+ * that ckd_knn launches for this tree (the main kernel only: not the query
+ * ordering nor the retry of short seeds).  Each call first reports and
+ * forgets the launches recorded so far -- *total_ms [host] their summed
+ * duration, *launches [host] their count (it synchronises those events;
+ * either pointer may be NULL) -- then applies `on`.
+ */
+int ckd_knn_timing(ckd_tree *tree, int32_t on, double *total_ms, int64_t *launches);
 
 /*
  * Ball search, first pass: offsets [dev, nq+1 int64] receives the CSR row
@@ -223,16 +235,39 @@ int ckd_mlp_nparam(void);
    grid [dev, G^3 int32], cell (x, y, z) at (z*G + y)*G + x. */
 int ckd_occupancy(const ckd_tree *tree, int32_t G, int32_t *grid, void *stream);
 
-/* Features of nq queries [dev, (nq, 3), prec] into X [dev, (nq, 32) bf16]:
-   (q + L/2)/L per axis, log1p of the 27 neighbouring grid counts (periodic),
-   two zero pads. */
-int ckd_mlp_features(const void *queries, int64_t nq, int32_t prec, const double *boxsize, int32_t G,
-                     const int32_t *grid, uint16_t *X, void *stream);
+/* bf16 log1p of every count of a G^3 grid: table [dev, G^3 uint16]. */
+int ckd_mlp_table(const int32_t *grid, int32_t G, uint16_t *table, void *stream);
+
+/* Features of nq queries [dev, (nq, 3), tree precision] against a periodic
+   3-D tree into X [dev, (nq, 32) bf16]: (q + L/2)/L per axis; the 27
+   log1p counts of the query's 3x3x3 periodic grid neighbourhood (dz, dy, dx
+   in -1..1, dz-major), looked up in `table` (ckd_mlp_table); then the local
+   density of the tree around the query: (1/3) ln(count / volume) of the
+   tight point box of the smallest ancestor of the query's leaf holding at
+   least 16 points, then at least 64 points (each box extent at least
+   1e-6 L).  leaf_of [dev, nq int32] or NULL: the query's leaf, when known
+   (otherwise found by descent). */
+int ckd_mlp_features(const ckd_tree *tree, const void *queries, int64_t nq, int32_t G, const uint16_t *table,
+                     const int32_t *leaf_of, uint16_t *X, void *stream);
 
 /* Forward pass: mode 0 writes y = predicted log radius, mode 1 writes
    scale * exp(y) (a seed radius for ckd_knn).  out [dev, n float]. */
 int ckd_mlp_forward(const float *params, const uint16_t *X, int64_t n, float scale, int32_t mode, float *out,
                     void *stream);
+
+/*
+ * kNN with learned seeds, in one call: the queries' leaves (found once, for
+ * both the query ordering and the tree features), the features
+ * (ckd_mlp_features with `table`, G), the network's seed radius
+ * margin * exp(y) per query (ckd_mlp_forward), then the seeded traversal.
+ * Same outputs and guarantee as ckd_knn: a query whose seed holds fewer than
+ * k points is searched again (first with a larger seed, then without), so
+ * the results never depend on the network.  Periodic 3-D trees, k <= 64.
+ * No reference counterpart (the reference has no learned component).
+ */
+int ckd_knn_learned(const ckd_tree *tree, const void *queries, int64_t nq, int32_t k, int32_t sorted,
+                    const float *params, float margin, int32_t G, const uint16_t *table, void *dist, int64_t *idx,
+                    int64_t *stats, void *stream);
 
 /* Mean-squared-error gradient over B rows against target [dev, B float]:
    grad [dev, nparam float] and *loss [dev] (sum of squared errors) are

@@ -6,12 +6,15 @@ in numpy float64, exactly what the GPU kernels in
 cosmokdtree_amd/csrc/ckd_mlp.hip compute, so their forward pass and
 gradients can be checked:
 
-* features: 30 per query = 3 x ((q + L/2) / L) followed by the 27
-  log1p(counts) of the 3x3x3 periodic neighbourhood of the query's cell in a
-  G^3 occupancy grid (G = 64), ordered dz-major (dz, dy, dx in -1..1), plus
-  2 zero pads (32 columns);
+* features: 32 per query = 3 x ((q + L/2) / L), the 27 log1p(counts) of
+  the 3x3x3 periodic neighbourhood of the query's cell in a G^3 occupancy
+  grid (G = 64), ordered dz-major (dz, dy, dx in -1..1), and 2 tree density
+  features (tree_features: (1/3) ln(count / volume) of the tight point box
+  of the smallest ancestor of the query's leaf holding >= 16, then >= 64
+  points; the leaf is found by the first descent of the kNN search,
+  cosmokdtree.f90:820-836, on the oracle's own tree);
 * network: y = w3 . gelu(W2^T gelu(W1^T x + b1) + b2) + b3, widths
-  32 -> 64 -> 64 -> 1, GELU(z) = z * 0.5 * (1 + erf(z / sqrt 2));
+  32 -> 64 -> 64 -> 1, GELU in its sigmoid form z * sigmoid(1.702 z);
 * precision: inputs, weights and hidden activations rounded to bfloat16
   (round to nearest even) before every product, products accumulated in
   float32 or wider, biases and the output in float32;
@@ -27,7 +30,7 @@ from __future__ import annotations
 from math import sqrt
 
 import numpy as np
-from scipy.special import erf
+from scipy.special import expit
 
 NIN, NH = 32, 64
 SIZES = dict(W1=NIN * NH, b1=NH, W2=NH * NH, b2=NH, w3=NH, b3=1)
@@ -61,19 +64,24 @@ def bf16(x):
     return r.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
+GELU_A = 1.702
+
+
 def gelu(z):
-    return z * 0.5 * (1.0 + erf(z / sqrt(2.0)))
+    """Sigmoid form of GELU, z * sigmoid(1.702 z) (Hendrycks & Gimpel 2016)."""
+    return z * expit(GELU_A * z)
 
 
 def gelu_grad(z):
-    return 0.5 * (1.0 + erf(z / sqrt(2.0))) + z * np.exp(-0.5 * z * z) / sqrt(2.0 * np.pi)
+    s = expit(GELU_A * z)
+    return s + GELU_A * z * s * (1.0 - s)
 
 
 def init_params(seed=0):
     rng = np.random.Generator(np.random.PCG64(seed))
     p = np.zeros(NPARAM, dtype=np.float32)
     w = unpack(p)
-    w["W1"][:30] = rng.standard_normal((30, NH)) * sqrt(2.0 / 30)
+    w["W1"][:] = rng.standard_normal((NIN, NH)) * sqrt(2.0 / NIN)
     w["W2"][:] = rng.standard_normal((NH, NH)) * sqrt(2.0 / NH)
     w["w3"][:] = rng.standard_normal(NH) * sqrt(1.0 / NH)
     return p
@@ -87,7 +95,49 @@ def occupancy(points, L, G=G_DEFAULT):
     return np.bincount(flat, minlength=G ** 3).astype(np.int32)
 
 
-def features(q, L, grid, G=G_DEFAULT):
+def tree_features(dump, q, L, thresholds=(16, 64)):
+    """Columns 30-31: density features from an oracle tree dump
+    (oracle.OracleTree.dump: preorder nodes, each with a contiguous range
+    [start, start + count) of the leaf-ordered points lpts).  `q` must be in
+    the tree's precision (the descent compares q[axis] < split in it)."""
+    q = np.asarray(q)
+    L = np.broadcast_to(np.asarray(L, np.float64), (3,))
+    leaf = dump["is_leaf"].astype(bool)
+    axis, split = dump["axis"], dump["split"]
+    left, right = dump["left"], dump["right"]
+    start, count, lpts = dump["start"], dump["count"], dump["lpts"]
+    parent = np.full(len(axis), -1, np.int64)
+    inner = np.nonzero(~leaf)[0]
+    parent[left[inner]] = inner
+    parent[right[inner]] = inner
+    v = np.zeros(len(q), np.int64)
+    rows = np.arange(len(q))
+    while True:  # first descent: left when q[axis] < split
+        act = ~leaf[v]
+        if not act.any():
+            break
+        va = v[act]
+        goleft = q[rows[act], axis[va]] < split[va]
+        v[act] = np.where(goleft, left[va], right[va])
+    out = np.zeros((len(q), 2), np.float32)
+    for col, c in enumerate(thresholds):
+        while True:
+            up = (count[v] < c) & (parent[v] >= 0)
+            if not up.any():
+                break
+            v[up] = parent[v[up]]
+        for node in np.unique(v):
+            sel = v == node
+            pts = lpts[start[node]:start[node] + count[node]].astype(np.float64)
+            ext = np.maximum(pts.max(axis=0) - pts.min(axis=0), 1e-6 * L)
+            V = (ext[0] * ext[1]) * ext[2]
+            out[sel, col] = np.float32(np.log(count[node] / V) / 3.0)
+    return out
+
+
+def features(q, L, grid, G=G_DEFAULT, tree_cols=None):
+    """The 32 feature columns; tree_cols = tree_features(...) fills 30-31
+    (zero otherwise)."""
     q = np.asarray(q, np.float64)
     f = np.zeros((len(q), NIN), dtype=np.float32)
     f[:, :3] = ((q + L / 2) / L).astype(np.float32)
@@ -101,6 +151,8 @@ def features(q, L, grid, G=G_DEFAULT):
                 z = (c[:, 2] + dz) % G
                 f[:, j] = np.log1p(grid[(z * G + y) * G + x].astype(np.float64)).astype(np.float32)
                 j += 1
+    if tree_cols is not None:
+        f[:, 30:32] = tree_cols
     return f
 
 

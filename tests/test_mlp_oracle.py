@@ -62,3 +62,46 @@ def test_features_definition():
     assert np.isclose(f[0, 3], np.log1p(grid[(3 * G + 3) * G + 3]))
     assert np.isclose(f[0, 3 + 13], np.log1p(grid[0]))  # centre cell
     assert np.all(f[:, 30:] == 0)
+
+
+def test_tree_features_definition():
+    """(1/3) ln(count / volume) of the tight box of the >=16 / >=64 ancestor:
+    with fewer than 16 points the root is that ancestor (checked by brute
+    force); scaling points and box by 2 (exact in floating point, same tree)
+    shifts every feature by -ln 2."""
+    from oracle.oracle import OracleTree
+    rng = np.random.default_rng(3)
+    pts = (rng.random((12, 3)) - 0.5) * 10
+    orc = OracleTree(pts, 4, np.full(3, 10.0))
+    try:
+        tf = M.tree_features(orc.dump(), pts[:5], 10.0)
+    finally:
+        orc.close()
+    ext = pts.max(0) - pts.min(0)
+    assert np.allclose(tf, np.log(12 / np.prod(ext)) / 3, rtol=0, atol=1e-6)
+    pts = (rng.random((3000, 3)) - 0.5) * 10
+    q = pts[:300] + rng.normal(0, 0.05, (300, 3))
+    q = np.clip(q, -5, 5)
+    o1, o2 = OracleTree(pts, 16, np.full(3, 10.0)), OracleTree(pts * 2, 16, np.full(3, 20.0))
+    try:
+        f1 = M.tree_features(o1.dump(), q, 10.0)
+        f2 = M.tree_features(o2.dump(), q * 2, 20.0)
+    finally:
+        o1.close()
+        o2.close()
+    assert np.allclose(f2 - f1, -np.log(2), atol=2e-6)
+    assert np.all(f1[:, 1] <= f1[:, 0] + 1.5)  # larger ancestors are not much denser
+
+
+def test_gelu_sigmoid_form():
+    """z * sigmoid(1.702 z): exact at 0 and in the tails, within 0.021 of
+    the erf form of GELU everywhere, and its derivative by differences."""
+    from scipy.special import erf
+    z = np.linspace(-6, 6, 2001)
+    exact = z * 0.5 * (1 + erf(z / np.sqrt(2)))
+    assert M.gelu(np.array([0.0]))[0] == 0.0
+    assert abs(M.gelu(np.array([1.0]))[0] - 1 / (1 + np.exp(-1.702))) < 1e-15
+    assert np.max(np.abs(M.gelu(z) - exact)) < 0.021
+    h = 1e-6
+    fd = (M.gelu(z + h) - M.gelu(z - h)) / (2 * h)
+    assert np.allclose(M.gelu_grad(z), fd, atol=1e-7)
