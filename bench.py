@@ -72,7 +72,7 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-seed path")
     ap.add_argument("--train-pairs", type=int, default=2_000_000)
     ap.add_argument("--epochs", type=int, default=4)
-    ap.add_argument("--quantile", type=float, default=0.98)
+    ap.add_argument("--quantile", type=float, default=0.95)
     ap.add_argument("--cpu-n", type=int, default=1 << 22)
     ap.add_argument("--cpu-q", type=int, default=1 << 18)
     ap.add_argument("--calib-copy-mib", type=int, default=0,

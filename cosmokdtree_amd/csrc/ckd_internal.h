@@ -171,6 +171,20 @@ __device__ __forceinline__ int64_t load_idx(const void *idx, int idx64, int64_t 
 // ----------------------------------------------------- wave / block scans
 constexpr int WAVE = 64;
 
+// XCD-aware block order.  Workgroups are dealt to the 8 XCDs (each with its
+// own L2) round-robin by block id.  This maps the blocks an XCD receives
+// within each group of 8 * XCD_RUN consecutive ids to a run of XCD_RUN
+// consecutive logical blocks, so neighbouring work (queries in leaf order)
+// shares an L2, while runs still rotate over the XCDs (balanced finish).
+// A bijection of [0, gridDim.x); a trailing partial group keeps its order.
+constexpr unsigned NXCD = 8, XCD_RUN = 32;
+__device__ __forceinline__ unsigned xcd_block() {
+    const unsigned b = blockIdx.x, grp = NXCD * XCD_RUN;
+    if (b >= gridDim.x / grp * grp) return b;
+    const unsigned w = b % grp;
+    return b - w + (w % NXCD) * XCD_RUN + w / NXCD;
+}
+
 template <typename V>
 __device__ __forceinline__ V wave_inclusive_scan(V v) {
     const int lane = threadIdx.x & (WAVE - 1);

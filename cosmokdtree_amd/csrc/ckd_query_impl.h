@@ -125,9 +125,10 @@ __global__ __launch_bounds__(QT, 3) void k_knn_w(TreeView<T> tv, const T *querie
                                               Retry rt) {
     __shared__ T buf_d[CBUF * QT];
     __shared__ int32_t buf_s[CBUF * QT];
-    const int64_t t = (int64_t)blockIdx.x * QT + threadIdx.x;
+    const int64_t blk = xcd_block();
+    const int64_t t = blk * QT + threadIdx.x;
     const int64_t n = nq_dev ? *nq_dev : nq;
-    if ((int64_t)blockIdx.x * QT + (threadIdx.x & ~(WAVE - 1)) >= n) return;  // whole wave idle
+    if (blk * QT + (threadIdx.x & ~(WAVE - 1)) >= n) return;  // whole wave idle
     Counters c;
     const bool active = t < n;
     const int64_t qi = active ? (order ? (int64_t)order[t] : t) : 0;
@@ -163,9 +164,10 @@ __global__ __launch_bounds__(QT, 3) void k_knn_s(TreeView<T> tv, int2 root, cons
     __shared__ T buf_d[CB > 0 ? CB * QT : 1];
     __shared__ int32_t buf_s[CB > 0 ? CB * QT : 1];
     __shared__ StackEntry<T, D> stk[QT / WAVE][SD];
-    const int64_t t = (int64_t)blockIdx.x * QT + threadIdx.x;
+    const int64_t blk = xcd_block();
+    const int64_t t = blk * QT + threadIdx.x;
     const int64_t n = nq_dev ? *nq_dev : nq;
-    if ((int64_t)blockIdx.x * QT + (threadIdx.x & ~(WAVE - 1)) >= n) return;  // whole wave idle
+    if (blk * QT + (threadIdx.x & ~(WAVE - 1)) >= n) return;  // whole wave idle
     Counters c;
     const bool active = t < n;
     const int64_t qi = active ? (order ? (int64_t)order[t] : t) : 0;
