@@ -15,7 +15,7 @@
 // W2[64][64] (row = input unit), b2[64], w3[64], b3.
 #include <cmath>
 
-#include "ckd_traverse.h"
+#include "ckd_features.h"
 
 namespace ckd {
 
@@ -31,12 +31,6 @@ constexpr int TR = 32;              // rows per wave tile
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ uint16_t f2bf(float x) {  // round to nearest even
-    uint32_t u = __float_as_uint(x);
-    u = u + 0x7FFFu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
-}
-__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
 // GELU in its sigmoid form, z * sigmoid(1.702 z) (Hendrycks & Gimpel,
 // "Gaussian Error Linear Units", 2016): one exp and one reciprocal
@@ -70,9 +64,19 @@ __device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
-struct MlpLds {
+// LDS of the forward-only kernel (37 KB: 4 blocks per CU) ...
+struct MlpLdsF {
     uint16_t w1t[NH][NIN];   // W1 transposed: w1t[n][k] = W1[k][n]
     uint16_t w2t[NH][NH];    // w2t[o][i] = W2[i][o]
+    float b1[NH], b2[NH], w3[NH], b3;
+    uint16_t x[WPB][TR][NIN];
+    uint16_t h1[WPB][TR][NH];
+};
+// ... and of the training kernel, which also needs W2 untransposed and a
+// backward buffer.
+struct MlpLds {
+    uint16_t w1t[NH][NIN];
+    uint16_t w2t[NH][NH];
     uint16_t w2r[NH][NH];    // w2r[i][o] = W2[i][o]
     float b1[NH], b2[NH], w3[NH], b3;
     uint16_t x[WPB][TR][NIN];
@@ -80,7 +84,9 @@ struct MlpLds {
     uint16_t h2[WPB][TR][NH];  // backward: dz2, then dz1
 };
 
-__device__ void load_params(MlpLds &s, const float *P, bool need_w2r) {
+template <typename S>
+__device__ void load_params(S &s, const float *P) {
+    constexpr bool need_w2r = sizeof(S) == sizeof(MlpLds);
     for (int e = threadIdx.x; e < NIN * NH; e += MT) {
         const int k = e / NH, n = e % NH;
         s.w1t[n][k] = f2bf(P[OW1 + e]);
@@ -89,7 +95,7 @@ __device__ void load_params(MlpLds &s, const float *P, bool need_w2r) {
         const int i = e / NH, o = e % NH;
         const uint16_t v = f2bf(P[OW2 + e]);
         s.w2t[o][i] = v;
-        if (need_w2r) s.w2r[i][o] = v;
+        if constexpr (need_w2r) s.w2r[i][o] = v;
     }
     for (int e = threadIdx.x; e < NH; e += MT) {
         s.b1[e] = P[OB1 + e];
@@ -122,7 +128,8 @@ struct FwdState {
     float y[2][4];
 };
 
-__device__ __forceinline__ void forward_tile(MlpLds &s, int w, FwdState &st) {
+template <typename S>
+__device__ __forceinline__ void forward_tile(S &s, int w, FwdState &st) {
     const int l = threadIdx.x & (WAVE - 1), r = l & 15, g = l >> 4;
 #pragma unroll
     for (int h = 0; h < 2; h++) {
@@ -173,8 +180,8 @@ __device__ __forceinline__ void forward_tile(MlpLds &s, int w, FwdState &st) {
 // mode 0: out = y (log r); mode 1: out = scale * exp(y) (a seed radius)
 __global__ __launch_bounds__(MT) void k_mlp_forward(const float *P, const uint16_t *X, int64_t n, float scale,
                                                     int mode, float *out) {
-    __shared__ MlpLds s;
-    load_params(s, P, false);
+    __shared__ MlpLdsF s;
+    load_params(s, P);
     const int w = threadIdx.x / WAVE, l = threadIdx.x & (WAVE - 1), r = l & 15, g = l >> 4;
     const int64_t ntile = ceil_div(n, TR);
     const int64_t iters = ceil_div(ntile, (int64_t)gridDim.x * WPB);
@@ -203,7 +210,7 @@ __global__ __launch_bounds__(MT) void k_mlp_forward(const float *P, const uint16
 __global__ __launch_bounds__(MT) void k_mlp_grad(const float *P, const uint16_t *X, const float *t, int64_t B,
                                                  float *grad, float *loss) {
     __shared__ MlpLds s;
-    load_params(s, P, true);
+    load_params(s, P);
     const int w = threadIdx.x / WAVE, l = threadIdx.x & (WAVE - 1), r = l & 15, g = l >> 4;
     f32x4 gw2[4][4], gw1[2][4];
     float gb1[4], gb2[4], gw3[4], gb3 = 0, lsum = 0;
@@ -393,61 +400,18 @@ __global__ void k_logtab(const int32_t *grid, int64_t cells, uint16_t *tab) {
     if (i < cells) tab[i] = f2bf((float)log1p((double)grid[i]));
 }
 
-// (1/3) ln(count / volume) of a node's tight point box, each extent at
-// least 1e-6 L: the log inverse length scale of the local point density.
-template <typename T>
-__device__ __forceinline__ float density_feature(const TreeView<T> &tv, int32_t v, const double *Ls) {
-    const T *b = tv.dbox + (int64_t)v * 6;
-    double V = 1.0;
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-        const double e = (double)b[3 + d] - (double)b[d], m = 1e-6 * Ls[d];
-        V = V * (e > m ? e : m);
-    }
-    return (float)(log((double)tv.range[v].count / V) / 3.0);
-}
-
-// 32 bf16 features per query: 3 positions (q + L/2) / L, the 27 log1p
-// counts of the query's 3x3x3 periodic cell neighbourhood (dz-major), and
-// the density feature of the smallest ancestor of the query's leaf (leaf
-// included) holding at least 16 points, then at least 64 points.
+// The 32 bf16 features of each query (query_features).
 template <typename T>
 __global__ void k_features(TreeView<T> tv, const T *q, int64_t nq, double L0, double L1, double L2, int G,
                            const uint16_t *tab, const int32_t *leaf_of, uint16_t *X) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nq) return;
     const double Ls[3] = {L0, L1, L2};
-    uint16_t f[NIN];
-    int c[3];
     T qq[3];
 #pragma unroll
-    for (int d = 0; d < 3; d++) {
-        qq[d] = q[i * 3 + d];
-        const double u = ((double)qq[d] + Ls[d] / 2) / Ls[d];
-        f[d] = f2bf((float)u);
-        int v = (int)floor(u * G);
-        c[d] = v < 0 ? 0 : (v > G - 1 ? G - 1 : v);
-    }
-    int j = 3;
-#pragma unroll
-    for (int dz = -1; dz <= 1; dz++)
-#pragma unroll
-        for (int dy = -1; dy <= 1; dy++)
-#pragma unroll
-            for (int dx = -1; dx <= 1; dx++) {
-                const int x = (c[0] + dx + G) % G, y = (c[1] + dy + G) % G, z = (c[2] + dz + G) % G;
-                f[j++] = tab[((int64_t)z * G + y) * G + x];
-            }
-    int32_t v = leaf_of ? leaf_of[i] : find_leaf<T, 3>(tv, qq, 0);
-    while (tv.range[v].count < 16 && tv.nodes[v].parent >= 0) v = tv.nodes[v].parent;
-    f[30] = f2bf(density_feature<T>(tv, v, Ls));
-    while (tv.range[v].count < 64 && tv.nodes[v].parent >= 0) v = tv.nodes[v].parent;
-    f[31] = f2bf(density_feature<T>(tv, v, Ls));
-    uint4 *dst = (uint4 *)(X + i * NIN);
-#pragma unroll
-    for (int w = 0; w < 4; w++)
-        dst[w] = make_uint4(f[8 * w] | ((uint32_t)f[8 * w + 1] << 16), f[8 * w + 2] | ((uint32_t)f[8 * w + 3] << 16),
-                            f[8 * w + 4] | ((uint32_t)f[8 * w + 5] << 16), f[8 * w + 6] | ((uint32_t)f[8 * w + 7] << 16));
+    for (int d = 0; d < 3; d++) qq[d] = q[i * 3 + d];
+    const int32_t leaf = leaf_of ? leaf_of[i] : find_leaf<T, 3>(tv, qq, 0);
+    query_features<T>(tv, qq, leaf, Ls, G, tab, X + i * NIN);
 }
 
 unsigned grid_blocks(int64_t rows) {

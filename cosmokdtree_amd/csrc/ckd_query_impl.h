@@ -5,7 +5,7 @@
 #pragma once
 #include <cstdlib>
 
-#include "ckd_traverse.h"
+#include "ckd_features.h"
 
 namespace ckd {
 
@@ -28,6 +28,24 @@ __global__ __launch_bounds__(QT) void k_locate(TreeView<T> tv, const T *queries,
     const int32_t leaf = find_leaf<T, D>(tv, q, 0);
     leaf_of[qi] = leaf;
     rank[qi] = (int32_t)atomicAdd((unsigned long long *)&leaf_cnt[leaf], 1ull);
+}
+
+// k_locate plus the learned features of each query (3-D periodic trees),
+// computed while the leaf is at hand.
+template <typename T>
+__global__ __launch_bounds__(QT) void k_locate_feat(TreeView<T> tv, const T *queries, int64_t nq, int32_t *leaf_of,
+                                                    int32_t *rank, int64_t *leaf_cnt, double L0, double L1,
+                                                    double L2, int G, const uint16_t *tab, uint16_t *X) {
+    const int64_t qi = (int64_t)blockIdx.x * QT + threadIdx.x;
+    if (qi >= nq) return;
+    T q[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) q[d] = queries[qi * 3 + d];
+    const int32_t leaf = find_leaf<T, 3>(tv, q, 0);
+    leaf_of[qi] = leaf;
+    rank[qi] = (int32_t)atomicAdd((unsigned long long *)&leaf_cnt[leaf], 1ull);
+    const double Ls[3] = {L0, L1, L2};
+    query_features<T>(tv, q, leaf, Ls, G, tab, X + qi * 32);
 }
 
 __global__ void k_order(const int32_t *leaf_of, const int32_t *rank, const int64_t *leaf_off, int64_t nq,
@@ -444,6 +462,16 @@ int knn_launch_k(const ckd_tree *t, const TreeView<T> &tv, int kind, int sd, con
     return CKD_OK;
 }
 
+// Workspace slot 2: the features (nq x 32 bf16) and seeds (nq floats).
+inline int learned_buffers(const ckd_tree *t, int64_t nq, uint16_t **X, float **seeds, hipStream_t s) {
+    void *ws = nullptr;
+    const int rc = workspace(const_cast<ckd_tree *>(t), 2, (size_t)nq * (64 + 4) + 256, &ws, s);
+    if (rc) return rc;
+    *X = (uint16_t *)ws;
+    *seeds = (float *)((char *)ws + (size_t)nq * 64);
+    return CKD_OK;
+}
+
 template <typename T, int D, bool PER>
 int knn_launch(const ckd_tree *t, const T *q, int64_t nq, int k, int sorted, const float *seed, const Learned *ln,
                T *dist, int64_t *idx, int64_t *stats, hipStream_t s) {
@@ -458,6 +486,9 @@ int knn_launch(const ckd_tree *t, const T *q, int64_t nq, int k, int sorted, con
         if (kind == CKD_KNN_STACK && !stack_ok<T, D>(t, sd)) kind = CKD_KNN_WAVE;  // same results, any depth
         if (D > 3) kind = CKD_KNN_LANE;  // the wave kernels are built for ndim <= 3
         const int32_t *order = nullptr, *leaf_of = nullptr;
+        uint16_t *X = nullptr;
+        float *seeds = nullptr;
+        bool feats_done = false;
         if (nq >= ORDER_MIN_QUERIES && nq < INT32_MAX && t->nnodes > 1) {
             // workspace: leaf_of, rank, order (int32 x nq), leaf counts and
             // offsets (int64 x nnodes+1), scan scratch
@@ -470,7 +501,15 @@ int knn_launch(const ckd_tree *t, const T *q, int64_t nq, int k, int sorted, con
             int64_t *off = cnt + t->nnodes + 1;
             void *scr = off + t->nnodes + 1;
             k_fill_i64<<<(unsigned)ceil_div(t->nnodes, 256), 256, 0, s>>>(cnt, t->nnodes, 0);
-            k_locate<T, D><<<g, QT, 0, s>>>(tv, q, nq, lo, rk, cnt);
+            if constexpr (D == 3) {
+                if (ln) {  // learned: the features come with the leaves
+                    if ((rc = learned_buffers(t, nq, &X, &seeds, s))) return rc;
+                    k_locate_feat<T><<<g, QT, 0, s>>>(tv, q, nq, lo, rk, cnt, t->L[0], t->L[1], t->L[2], ln->G,
+                                                      ln->table, X);
+                    feats_done = true;
+                }
+            }
+            if (!feats_done) k_locate<T, D><<<g, QT, 0, s>>>(tv, q, nq, lo, rk, cnt);
             if ((rc = scan_i64(cnt, off, t->nnodes, false, scr, s))) return rc;
             k_order<<<(unsigned)ceil_div(nq, 256), 256, 0, s>>>(lo, rk, off, nq, ord);
             CKD_HIP(hipGetLastError());
@@ -478,16 +517,15 @@ int knn_launch(const ckd_tree *t, const T *q, int64_t nq, int k, int sorted, con
             leaf_of = lo;
         }
         if (ln) {
-            // learned seeds: features (reusing the leaves just found) and the
-            // network, into workspace slot 2
-            void *ws = nullptr;
-            int rc = workspace(const_cast<ckd_tree *>(t), 2, (size_t)nq * (64 + 4) + 256, &ws, s);
-            if (rc) return rc;
-            uint16_t *X = (uint16_t *)ws;
-            float *sd_out = (float *)((char *)ws + (size_t)nq * 64);
-            if ((rc = ckd_mlp_features(t, q, nq, ln->G, ln->table, leaf_of, X, s))) return rc;
-            if ((rc = ckd_mlp_forward(ln->params, X, nq, ln->margin, 1, sd_out, s))) return rc;
-            seed = sd_out;
+            // learned seeds: the features (unless the ordering pass made
+            // them), then the network
+            int rc;
+            if (!feats_done) {
+                if ((rc = learned_buffers(t, nq, &X, &seeds, s))) return rc;
+                if ((rc = ckd_mlp_features(t, q, nq, ln->G, ln->table, leaf_of, X, s))) return rc;
+            }
+            if ((rc = ckd_mlp_forward(ln->params, X, nq, ln->margin, 1, seeds, s))) return rc;
+            seed = seeds;
         }
         if (k <= 8) return knn_launch_k<T, D, PER, 8>(t, tv, kind, sd, q, nq, k, sorted, seed, order, leaf_of, dist, idx, stats, s);
         if (k <= 16) return knn_launch_k<T, D, PER, 16>(t, tv, kind, sd, q, nq, k, sorted, seed, order, leaf_of, dist, idx, stats, s);
