@@ -52,7 +52,7 @@ def _load():
     lib.ckd_free.argtypes = [vp]
     lib.ckd_get_info.argtypes = [vp, C.POINTER(TreeInfo)]
     lib.ckd_knn.argtypes = [vp, vp, i64, i32, i32, vp, vp, vp, vp, vp]
-    lib.ckd_knn_config.argtypes = [vp, i32, i32]
+    lib.ckd_knn_config.argtypes = [vp, i32, i32, i32]
     lib.ckd_knn_timing.argtypes = [vp, i32, C.POINTER(C.c_double), C.POINTER(i64)]
     lib.ckd_ball_count.argtypes = [vp, vp, i64, dbl, vp, C.POINTER(i64), vp]
     lib.ckd_ball_fill.argtypes = [vp, vp, i64, dbl, i32, vp, vp, vp, vp]

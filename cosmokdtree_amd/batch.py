@@ -29,7 +29,8 @@ lib.ckd_batch_plan.argtypes = [_vp_t, _vp_t, _vp_t, _vp_t, C.c_int32, C.c_int32,
 lib.ckd_batch_run.argtypes = [_vp_t] * 8
 lib.ckd_batch_status.argtypes = [_vp_t, _vp_t, C.POINTER(C.c_int64), C.POINTER(C.c_int32), _vp_t]
 lib.ckd_batch_free.argtypes = [_vp_t]
-for _n in ("ckd_batch_plan", "ckd_batch_run", "ckd_batch_status", "ckd_batch_free"):
+lib.ckd_batch_config.argtypes = [_vp_t, C.c_int32, C.c_int32, C.c_int32]
+for _n in ("ckd_batch_plan", "ckd_batch_run", "ckd_batch_status", "ckd_batch_free", "ckd_batch_config"):
     getattr(lib, _n).restype = C.c_int
 
 
@@ -67,8 +68,10 @@ class BatchPlan:
     """Work queue and output layout for one batched query workload."""
 
     def __init__(self, forest: KDTree, nq_per_set, k_per_set, radius_per_set=None, sorted: bool = True,
-                 threads: int = 256, blocks: int = 0, ball_capacity: int | None = None):
+                 threads: int = 256, blocks: int = 0, ball_capacity: int | None = None, knn_kernel: str | None = None,
+                 queries_per_wave: int = 64, stack_depth: int = 0):
         self.forest = forest
+        self.knn_kernel, self.queries_per_wave, self.stack_depth = knn_kernel, int(queries_per_wave), int(stack_depth)
         nq = np.asarray(nq_per_set, dtype=np.int64)
         self.qoff = np.zeros(len(nq) + 1, dtype=np.int64)
         np.cumsum(nq, out=self.qoff[1:])
@@ -94,6 +97,7 @@ class BatchPlan:
                                  int(self.sorted), self.threads, self.blocks, int(capacity), C.byref(plan),
                                  C.byref(tot)), "batch_plan")
         self._plan = plan
+        self._apply_config()
         self.knn_total = tot.value
         self.capacity = int(capacity)
         dev, dt = self.forest.device, self.forest.torch_dtype
@@ -102,6 +106,25 @@ class BatchPlan:
         self.ball_off = torch.empty(self.nq + 1, dtype=torch.int64, device=dev)
         self.ball_dist = torch.empty(max(self.capacity, 1), dtype=dt, device=dev)
         self.ball_idx = torch.empty(max(self.capacity, 1), dtype=torch.int64, device=dev)
+
+    def _apply_config(self):
+        if self.knn_kernel is None:  # the plan's own default (by query density)
+            return
+        kinds = {"lane": 0, "stack": 2}
+        if self.knn_kernel not in kinds:
+            raise ValueError("knn_kernel must be 'lane' or 'stack'")
+        check(lib.ckd_batch_config(self._plan, kinds[self.knn_kernel], self.queries_per_wave, self.stack_depth),
+              "batch_config")
+
+    def config(self, knn_kernel: str = "stack", queries_per_wave: int = 64, stack_depth: int = 0):
+        """kNN kernel ("stack": shared walks, queries in leaf order, or
+        "lane": one walk per query), queries per wave of the stack walk
+        (1, 4, 16, 64) and its stack depth (0, 32, 64).  Results do not
+        depend on it."""
+        self.knn_kernel, self.queries_per_wave, self.stack_depth = knn_kernel, int(queries_per_wave), int(stack_depth)
+        self._apply_config()
+        self._graph = None
+        return self
 
     def __del__(self):
         if getattr(self, "_plan", None):

@@ -292,7 +292,7 @@ int ckd_knn_learned(const ckd_tree *t, const void *queries, int64_t nq, int32_t 
                              (hipStream_t)stream);
 }
 
-int ckd_knn_config(ckd_tree *t, int32_t kernel, int32_t stack_depth) {
+int ckd_knn_config(ckd_tree *t, int32_t kernel, int32_t stack_depth, int32_t queries_per_wave) {
     g_err[0] = 0;
     if (!t) {
         set_error("tree must not be NULL");
@@ -306,8 +306,14 @@ int ckd_knn_config(ckd_tree *t, int32_t kernel, int32_t stack_depth) {
         set_error("stack depth must be 0, 32 or 64, got %d", stack_depth);
         return CKD_ERR_ARG;
     }
+    if (queries_per_wave != 0 && queries_per_wave != 1 && queries_per_wave != 4 && queries_per_wave != 16 &&
+        queries_per_wave != 64) {
+        set_error("queries per wave must be 0, 1, 4, 16 or 64, got %d", queries_per_wave);
+        return CKD_ERR_ARG;
+    }
     t->knn_kernel = kernel;
     t->knn_stack = stack_depth;
+    t->knn_qpw = queries_per_wave;
     return CKD_OK;
 }
 

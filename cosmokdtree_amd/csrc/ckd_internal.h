@@ -254,6 +254,7 @@ struct ckd_tree {
     int2 *wroot = nullptr;   // device [nsets] root pairs
     int knn_kernel = -1;     // kNN kernel choice, -1 = automatic (ckd_knn_config)
     int knn_stack = 0;       // stack depth of the stack kernel, 0 = automatic
+    int knn_qpw = 0;         // queries per wave of the stack kernel (1..64), 0 = 64
     int timing = 0;          // record events around kNN traversal launches (ckd_knn_timing)
     std::vector<std::pair<hipEvent_t, hipEvent_t>> knn_events;
     // grow-only query workspaces (calls on one tree must not overlap):

@@ -173,19 +173,22 @@ __global__ __launch_bounds__(QT, 3) void k_knn_w(TreeView<T> tv, const T *querie
 // Candidate buffering pays from k = 16 on (below, the short list update is
 // cheaper than the LDS round trip and the occupancy the buffer costs); 8
 // slots per lane in double precision keep the LDS within 4 blocks per CU.
+// G lanes per query (1, 2, 4, ..., 64; 64 / G queries per wave): the grid
+// covers nq * G lanes.
 template <typename T, int D, bool PER, int KMAX, int SD>
 __global__ __launch_bounds__(QT, 3) void k_knn_s(TreeView<T> tv, int2 root, const T *queries, int64_t nq,
                                               const int64_t *nq_dev, int k, int sorted, const float *seed,
                                               const int32_t *order, T *dout, int64_t *iout, int64_t *stats,
-                                              Retry rt) {
+                                              Retry rt, int G) {
     constexpr int CB = KMAX < 16 ? 0 : (sizeof(T) == 4 ? 16 : 8);
     __shared__ T buf_d[CB > 0 ? CB * QT : 1];
     __shared__ int32_t buf_s[CB > 0 ? CB * QT : 1];
     __shared__ StackEntry<T, D> stk[QT / WAVE][SD];
     const int64_t blk = xcd_block();
-    const int64_t t = blk * QT + threadIdx.x;
+    const int lane = threadIdx.x & (WAVE - 1), sub = lane & (G - 1), gbase = lane & ~(G - 1);
+    const int64_t t = (blk * QT + threadIdx.x) / G;
     const int64_t n = nq_dev ? *nq_dev : nq;
-    if (blk * QT + (threadIdx.x & ~(WAVE - 1)) >= n) return;  // whole wave idle
+    if ((blk * QT + (threadIdx.x & ~(WAVE - 1))) / G >= n) return;  // whole wave idle
     Counters c;
     const bool active = t < n;
     const int64_t qi = active ? (order ? (int64_t)order[t] : t) : 0;
@@ -201,11 +204,11 @@ __global__ __launch_bounds__(QT, 3) void k_knn_s(TreeView<T> tv, int2 root, cons
     int found = 0;
     const bool done = knn_stack<T, D, PER, KMAX, SD, CB>(tv, root, q, active, k, sorted, seed2, dout + qi * k,
                                                           iout + qi * k, 0, c, buf_d, buf_s, threadIdx.x, QT,
-                                                          stk[threadIdx.x / WAVE], found);
-    if (!done) send_retry(rt, t, qi, sr, found, k);
+                                                          stk[threadIdx.x / WAVE], found, G, sub, gbase);
+    if (!done && sub == 0) send_retry(rt, t, qi, sr, found, k);
     if (stats) {
-        if (!active) c = Counters();
-        flush_stats(c, active && !nq_dev ? 1 : 0, stats);
+        if (!active || sub != 0) c = Counters();
+        flush_stats(c, active && sub == 0 && !nq_dev ? 1 : 0, stats);
     }
 }
 
@@ -360,18 +363,20 @@ struct Learned {
 };
 
 template <typename T, int D, bool PER, int KMAX>
-int launch_wave(const ckd_tree *t, const TreeView<T> &tv, int kind, int sd, unsigned g, const T *q, int64_t nq,
-                const int64_t *nq_dev, int k, int sorted, const float *seed, const int32_t *order, T *dist,
-                int64_t *idx, int64_t *stats, const Retry &rt, hipStream_t s) {
+int launch_wave(const ckd_tree *t, const TreeView<T> &tv, int kind, int sd, int qpw, unsigned g, const T *q,
+                int64_t nq, const int64_t *nq_dev, int k, int sorted, const float *seed, const int32_t *order,
+                T *dist, int64_t *idx, int64_t *stats, const Retry &rt, hipStream_t s) {
     if (kind == CKD_KNN_STACK) {
         if constexpr (D <= 3) {
             const int2 root = make_int2(0, t->nnodes > 1 ? -1 : (int32_t)t->n);
+            const int G = WAVE / qpw;
+            const unsigned gs = (unsigned)ceil_div(nq * G, QT);
             if (sd <= 32)
-                k_knn_s<T, D, PER, KMAX, 32><<<g, QT, 0, s>>>(tv, root, q, nq, nq_dev, k, sorted, seed, order, dist,
-                                                               idx, stats, rt);
+                k_knn_s<T, D, PER, KMAX, 32><<<gs, QT, 0, s>>>(tv, root, q, nq, nq_dev, k, sorted, seed, order, dist,
+                                                                idx, stats, rt, G);
             else
-                k_knn_s<T, D, PER, KMAX, 64><<<g, QT, 0, s>>>(tv, root, q, nq, nq_dev, k, sorted, seed, order, dist,
-                                                               idx, stats, rt);
+                k_knn_s<T, D, PER, KMAX, 64><<<gs, QT, 0, s>>>(tv, root, q, nq, nq_dev, k, sorted, seed, order, dist,
+                                                                idx, stats, rt, G);
         }
     } else if constexpr (D <= 3) {
         k_knn_w<T, D, PER, KMAX><<<g, QT, 0, s>>>(tv, q, nq, nq_dev, k, sorted, seed, order, dist, idx, stats, rt);
@@ -400,7 +405,7 @@ struct LaunchTimer {
 };
 
 template <typename T, int D, bool PER, int KMAX>
-int knn_launch_k(const ckd_tree *t, const TreeView<T> &tv, int kind, int sd, const T *q, int64_t nq, int k,
+int knn_launch_k(const ckd_tree *t, const TreeView<T> &tv, int kind, int sd, int qpw, const T *q, int64_t nq, int k,
                  int sorted, const float *seed, const int32_t *order, const int32_t *leaf_of, T *dist, int64_t *idx,
                  int64_t *stats, hipStream_t s) {
     const unsigned g = (unsigned)ceil_div(nq, QT);
@@ -414,7 +419,7 @@ int knn_launch_k(const ckd_tree *t, const TreeView<T> &tv, int kind, int sd, con
     }
     if (!seed) {
         if ((rc = tm.start(s))) return rc;
-        if ((rc = launch_wave<T, D, PER, KMAX>(t, tv, kind, sd, g, q, nq, nullptr, k, sorted, nullptr, order, dist, idx,
+        if ((rc = launch_wave<T, D, PER, KMAX>(t, tv, kind, sd, qpw, g, q, nq, nullptr, k, sorted, nullptr, order, dist, idx,
                                                stats, Retry(), s)))
             return rc;
         return tm.stop(s);
@@ -445,7 +450,7 @@ int knn_launch_k(const ckd_tree *t, const TreeView<T> &tv, int kind, int sd, con
     r1.flag = flag;
     r1.seed_next = seed_next;
     if ((rc = tm.start(s))) return rc;
-    if ((rc = launch_wave<T, D, PER, KMAX>(t, tv, kind, sd, g, q, nq, nullptr, k, sorted, seed, order, dist, idx,
+    if ((rc = launch_wave<T, D, PER, KMAX>(t, tv, kind, sd, qpw, g, q, nq, nullptr, k, sorted, seed, order, dist, idx,
                                            stats, r1, s)))
         return rc;
     if ((rc = tm.stop(s))) return rc;
@@ -454,7 +459,7 @@ int knn_launch_k(const ckd_tree *t, const TreeView<T> &tv, int kind, int sd, con
     Retry r2;
     r2.list = list2;
     r2.count = count2;
-    if ((rc = launch_wave<T, D, PER, KMAX>(t, tv, kind, sd, g, q, nq, off + nq, k, sorted, seed_next, list, dist, idx,
+    if ((rc = launch_wave<T, D, PER, KMAX>(t, tv, kind, sd, qpw, g, q, nq, off + nq, k, sorted, seed_next, list, dist, idx,
                                            stats, r2, s)))
         return rc;
     k_knn<T, D, PER, KMAX><<<g, QT, 0, s>>>(tv, q, nq, count2, k, sorted, nullptr, list2, nullptr, dist, idx, stats);
@@ -482,7 +487,20 @@ int knn_launch(const ckd_tree *t, const T *q, int64_t nq, int k, int sorted, con
         int kind = t->knn_kernel >= 0 ? t->knn_kernel : knn_kernel_env();
         int sd = t->knn_stack > 0 ? t->knn_stack : (t->depth <= 32 ? 32 : 64);
         if (sd < t->depth) sd = 64;
-        if (kind < 0) kind = stack_ok<T, D>(t, 64) ? CKD_KNN_STACK : (k > 8 ? CKD_KNN_WAVE : CKD_KNN_LANE);
+        int qpw = t->knn_qpw > 0 ? t->knn_qpw : 64;
+        if (kind < 0) {
+            // Automatic choice (measured on MI355X): queries at least a quarter
+            // as many as the points share walks 64 per wave; a sparse batch
+            // small enough to leave the GPU half idle runs one query per
+            // wave (its 64 lanes split the leaf rows); a large sparse batch
+            // runs one walk per query.
+            if (!stack_ok<T, D>(t, 64)) kind = k > 8 ? CKD_KNN_WAVE : CKD_KNN_LANE;
+            else if (nq * 4 >= t->n) kind = CKD_KNN_STACK;
+            else if (nq <= 65536) {
+                kind = CKD_KNN_STACK;
+                if (t->knn_qpw == 0) qpw = 1;
+            } else kind = CKD_KNN_LANE;
+        }
         if (kind == CKD_KNN_STACK && !stack_ok<T, D>(t, sd)) kind = CKD_KNN_WAVE;  // same results, any depth
         if (D > 3) kind = CKD_KNN_LANE;  // the wave kernels are built for ndim <= 3
         const int32_t *order = nullptr, *leaf_of = nullptr;
@@ -527,10 +545,10 @@ int knn_launch(const ckd_tree *t, const T *q, int64_t nq, int k, int sorted, con
             if ((rc = ckd_mlp_forward(ln->params, X, nq, ln->margin, 1, seeds, s))) return rc;
             seed = seeds;
         }
-        if (k <= 8) return knn_launch_k<T, D, PER, 8>(t, tv, kind, sd, q, nq, k, sorted, seed, order, leaf_of, dist, idx, stats, s);
-        if (k <= 16) return knn_launch_k<T, D, PER, 16>(t, tv, kind, sd, q, nq, k, sorted, seed, order, leaf_of, dist, idx, stats, s);
-        if (k <= 32) return knn_launch_k<T, D, PER, 32>(t, tv, kind, sd, q, nq, k, sorted, seed, order, leaf_of, dist, idx, stats, s);
-        return knn_launch_k<T, D, PER, 64>(t, tv, kind, sd, q, nq, k, sorted, seed, order, leaf_of, dist, idx, stats, s);
+        if (k <= 8) return knn_launch_k<T, D, PER, 8>(t, tv, kind, sd, qpw, q, nq, k, sorted, seed, order, leaf_of, dist, idx, stats, s);
+        if (k <= 16) return knn_launch_k<T, D, PER, 16>(t, tv, kind, sd, qpw, q, nq, k, sorted, seed, order, leaf_of, dist, idx, stats, s);
+        if (k <= 32) return knn_launch_k<T, D, PER, 32>(t, tv, kind, sd, qpw, q, nq, k, sorted, seed, order, leaf_of, dist, idx, stats, s);
+        return knn_launch_k<T, D, PER, 64>(t, tv, kind, sd, qpw, q, nq, k, sorted, seed, order, leaf_of, dist, idx, stats, s);
     }
     if (ln) {
         set_error("learned seeds need k <= 64");

@@ -473,6 +473,37 @@ __device__ __forceinline__ void wave_scan_rows(const TreeView<T> &tv, int32_t pb
     c.tested += pe - pb;
 }
 
+// Leaf scan of a query held by a group of G lanes (G = 2..64, the lanes
+// gbase .. gbase+G-1 of the wave, this one being gbase+sub): each lane
+// computes the distance of one row of every G-row slice, then the group
+// inserts the slice's rows in row order, so every lane of the group keeps
+// the same list as a one-lane scan would.
+template <typename T, int D, bool PER, int KMAX>
+__device__ __forceinline__ void group_scan_rows(const TreeView<T> &tv, int32_t pb, int32_t pe, const T (&q)[D],
+                                                bool run, T (&bd)[KMAX], int32_t (&bs)[KMAX], Counters &c, int G,
+                                                int sub, int gbase) {
+#pragma unroll 1
+    for (int32_t base = pb; base < pe; base += G) {
+        const int32_t p = base + sub;
+        T d = T(INFINITY);  // rows past the leaf are never accepted
+        if (p < pe) {
+            T x[D];
+            load_pt<T, D>(tv.pts, p, x);
+            d = dist2<T, D, PER>(x, q, tv.L);
+        }
+        const int ns = pe - base < G ? pe - base : G;
+#pragma unroll 1
+        for (int j = 0; j < ns; j++) {
+            const T dj = __shfl(d, gbase + j, WAVE);
+            if (run && dj < bd[KMAX - 1]) {
+                list_insert<T, KMAX>(bd, bs, dj, base + j);
+                c.inserted++;
+            }
+        }
+    }
+    c.tested += pe - pb;
+}
+
 // Final rows of a kNN list: k (sqrt distance, index - idx_base) pairs,
 // equal distances in index order when `sorted`.
 template <typename T, int KMAX>
@@ -536,13 +567,15 @@ __device__ __forceinline__ T wrap_free_bound(const T (&q)[D], const T *L) {
 template <typename T, int D, bool PER, int KMAX, int SD, int CB, bool CHECK>
 __device__ __forceinline__ bool stack_walk(const TreeView<T> &tv, const T (&q)[D], bool run, T safe2, T (&bd)[KMAX],
                                            int32_t (&bs)[KMAX], int32_t &a, int32_t &b, int &sp, Counters &c,
-                                           T *buf_d, int32_t *buf_s, int tid, int nthreads, StackEntry<T, D> *stk) {
+                                           T *buf_d, int32_t *buf_s, int tid, int nthreads, StackEntry<T, D> *stk,
+                                           int G, int sub, int gbase) {
     for (;;) {
         if (CHECK && !__any(run && !(safe2 >= bd[KMAX - 1]))) return false;
         c.visited++;
         bool descend = false;
         if (b >= 0) {
-            wave_scan_rows<T, D, PER, KMAX, CB>(tv, a, a + b, q, run, bd, bs, c, buf_d, buf_s, tid, nthreads);
+            if (G == 1) wave_scan_rows<T, D, PER, KMAX, CB>(tv, a, a + b, q, run, bd, bs, c, buf_d, buf_s, tid, nthreads);
+            else group_scan_rows<T, D, PER, KMAX>(tv, a, a + b, q, run, bd, bs, c, G, sub, gbase);
         } else {
             T b0[2 * D], b1[2 * D];
             int4 r;
@@ -598,12 +631,14 @@ __device__ __forceinline__ bool stack_walk(const TreeView<T> &tv, const T (&q)[D
 // Returns false for a seeded lane whose seed held fewer than k points (it
 // writes nothing then and sets `found` to the number it held; the caller
 // searches it again with a larger or no seed).  `stk` is this wave's stack
-// of SD entries in LDS.
+// of SD entries in LDS.  With G > 1 each query is held by G lanes (lane
+// gbase + sub of its group; 64 / G queries per wave): they split the leaf
+// rows, keep identical lists, and lane sub == 0 writes the row.
 template <typename T, int D, bool PER, int KMAX, int SD, int CB>
 __device__ __forceinline__ bool knn_stack(const TreeView<T> &tv, int2 root, const T (&q)[D], bool active, int k,
                                           int sorted, T seed2, T *drow, int64_t *irow, int64_t idx_base, Counters &c,
                                           T *buf_d, int32_t *buf_s, int tid, int nthreads, StackEntry<T, D> *stk,
-                                          int &found) {
+                                          int &found, int G = 1, int sub = 0, int gbase = 0) {
     T bd[KMAX];
     int32_t bs[KMAX];
     const T init = seed2 < T(FLT_MAX) ? seed2 : T(FLT_MAX);
@@ -618,12 +653,12 @@ __device__ __forceinline__ bool knn_stack(const TreeView<T> &tv, int2 root, cons
     if constexpr (PER) {
         const T safe2 = wrap_free_bound<T, D>(q, tv.L);
         if (!stack_walk<T, D, true, KMAX, SD, CB, true>(tv, q, run, safe2, bd, bs, a, b, sp, c, buf_d, buf_s, tid,
-                                                        nthreads, stk))
+                                                        nthreads, stk, G, sub, gbase))
             stack_walk<T, D, false, KMAX, SD, CB, false>(tv, q, run, safe2, bd, bs, a, b, sp, c, buf_d, buf_s, tid,
-                                                         nthreads, stk);
+                                                         nthreads, stk, G, sub, gbase);
     } else {
         stack_walk<T, D, false, KMAX, SD, CB, false>(tv, q, run, T(0), bd, bs, a, b, sp, c, buf_d, buf_s, tid,
-                                                     nthreads, stk);
+                                                     nthreads, stk, G, sub, gbase);
     }
     if (!active) return true;
     if (bs[KMAX - 1] < 0 && init != T(FLT_MAX)) {
@@ -633,7 +668,7 @@ __device__ __forceinline__ bool knn_stack(const TreeView<T> &tv, int2 root, cons
         for (int j = 0; j < KMAX; j++) found += bs[j] >= 0 ? 1 : 0;
         return false;
     }
-    knn_write_row<T, KMAX>(tv, bd, bs, k, sorted, drow, irow, idx_base);
+    if (sub == 0) knn_write_row<T, KMAX>(tv, bd, bs, k, sorted, drow, irow, idx_base);
     return true;
 }
 

@@ -91,13 +91,15 @@ class KDTree(int):
             self._alive = False
             _REGISTRY.pop(int(self), None)
 
-    def knn_config(self, kernel: str = "auto", stack: int = 0):
-        """Pick the kNN kernel ("auto", "lane", "wave", "stack") and the
-        stack depth (0 = automatic, 32, 64).  Results do not depend on it."""
+    def knn_config(self, kernel: str = "auto", stack: int = 0, queries_per_wave: int = 0):
+        """Pick the kNN kernel ("auto", "lane", "wave", "stack"), the stack
+        depth (0 = automatic, 32, 64) and the queries sharing a wave of the
+        stack kernel (0 = 64, 1, 4, 16, 64).  Results do not depend on it."""
         self._check_alive()
         if kernel not in _lib.KNN_KERNELS:
             raise ValueError(f"kernel must be one of {sorted(_lib.KNN_KERNELS)}")
-        check(lib.ckd_knn_config(self._ptr, _lib.KNN_KERNELS[kernel], int(stack)), "knn_config")
+        check(lib.ckd_knn_config(self._ptr, _lib.KNN_KERNELS[kernel], int(stack), int(queries_per_wave)),
+              "knn_config")
         return self
 
     def knn_timing(self, on: bool = True):

@@ -10,6 +10,12 @@
   [-L/2, L/2] as a periodic tree requires (cosmokdtree.f90:176-190).
 * ``halo_set(n, L, seed)``: one Gaussian halo plus a 20% uniform background
   (the batched uneven-instance workload).
+* ``batched_sets(total, nsets, L, seed)``: the batched workload at scale:
+  nsets set sizes log-uniform in [1e3, 1e6] rescaled to sum to `total`,
+  each set a halo_set-style Gaussian halo (sigma = 0.02 n_halo^(1/3)) plus
+  20% uniform background in its own periodic box of side L; 10% of each
+  set's points as its queries, k per set from {8, 16, 32, 64}, a ball
+  radius per set log-uniform in [0.05, 0.5].
 
 All generators use numpy's PCG64 with the given seed; float64 results can
 be cast to float32 by the caller.
@@ -88,3 +94,39 @@ def halo_set(n: int, L: float = 10.0, seed: int = 0, dtype=np.float32, bg_frac: 
     h = dtype(L / 2)
     np.clip(pts, -h, h, out=pts)
     return pts
+
+
+def batched_sets(total: int = 200_000_000, nsets: int = 4096, L: float = 10.0, seed: int = SEED_POINTS,
+                 query_frac: float = 0.1):
+    """The batched workload, generated vectorised in float32.  Returns
+    (points (N, 3) float32, offsets (nsets+1) int64, queries (Q, 3) float32,
+    query offsets (nsets+1) int64, k (nsets) int32, radius (nsets) float64)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    raw = np.exp(rng.uniform(np.log(1e3), np.log(1e6), nsets))
+    sizes = np.maximum(np.floor(raw * total / raw.sum()).astype(np.int64), 1)
+    sizes[-1] += total - sizes.sum()
+    off = np.zeros(nsets + 1, dtype=np.int64)
+    np.cumsum(sizes, out=off[1:])
+    nb = np.round(sizes * 0.2).astype(np.int64)
+    nh = sizes - nb
+    centres = ((rng.random((nsets, 3)) - 0.5) * L).astype(np.float32)
+    sig = (0.02 * np.cbrt(np.maximum(nh, 1).astype(np.float64))).astype(np.float32)
+    set_of = np.repeat(np.arange(nsets), sizes)
+    local = np.arange(total, dtype=np.int64) - off[set_of]
+    halo = local >= nb[set_of]
+    pts = ((rng.random((total, 3), dtype=np.float32) - np.float32(0.5)) * np.float32(L))
+    g = rng.standard_normal((total, 3), dtype=np.float32)
+    hp = centres[set_of] + sig[set_of, None] * g
+    hp -= np.float32(L) * np.floor((hp + np.float32(L / 2)) / np.float32(L))
+    pts[halo] = hp[halo]
+    del g, hp
+    h = np.float32(L / 2)
+    np.clip(pts, -h, h, out=pts)
+    nq = np.maximum((sizes * query_frac).astype(np.int64), 1)
+    qoff = np.zeros(nsets + 1, dtype=np.int64)
+    np.cumsum(nq, out=qoff[1:])
+    qsel = off[np.repeat(np.arange(nsets), nq)] + (rng.random(int(qoff[-1])) * np.repeat(sizes, nq)).astype(np.int64)
+    queries = pts[qsel]
+    k = rng.choice(np.array([8, 16, 32, 64], dtype=np.int32), nsets)
+    radius = np.exp(rng.uniform(np.log(0.05), np.log(0.5), nsets))
+    return pts, off, queries, qoff, k, radius

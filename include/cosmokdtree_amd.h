@@ -123,10 +123,15 @@ int ckd_knn(const ckd_tree *tree, const void *queries, int64_t nq, int32_t k, in
  * the reference has a single CPU traversal.  kernel is one of CKD_KNN_*;
  * stack_depth is 0 (automatic), 32 or 64, the LDS stack entries per wave of
  * CKD_KNN_STACK, which is used only when ndim <= 3 and the tree depth fits
- * the stack (otherwise CKD_KNN_WAVE runs).  Every choice returns identical
- * results; only the speed differs.  Used by the autotuner.
+ * the stack (otherwise CKD_KNN_WAVE runs); queries_per_wave is 0 (= 64), 1,
+ * 4, 16 or 64: how many queries share one wave of CKD_KNN_STACK (each then
+ * held by 64 / queries_per_wave lanes that split the leaf rows).  With
+ * CKD_KNN_AUTO, a call with at least n/4 queries shares walks 64 per wave,
+ * a sparser one of at most 65536 queries runs one query per wave, a larger
+ * sparse one one walk per query.  Every choice returns identical results;
+ * only the speed differs.  Used by the autotuner.
  */
-int ckd_knn_config(ckd_tree *tree, int32_t kernel, int32_t stack_depth);
+int ckd_knn_config(ckd_tree *tree, int32_t kernel, int32_t stack_depth, int32_t queries_per_wave);
 
 /*
  * Launch timing of the kNN traversal kernel of one tree (no reference
@@ -216,6 +221,14 @@ typedef struct ckd_batch ckd_batch;
 int ckd_batch_plan(const ckd_tree *forest, const int64_t *qoff, const int32_t *k, const double *radius,
                    int32_t sorted, int32_t threads, int32_t blocks, int64_t ball_capacity, ckd_batch **out,
                    int64_t *knn_total);
+/* kNN kernel of a plan: CKD_KNN_STACK (the default when the queries are at
+   least half as many as the points; queries sorted by leaf
+   within each set, queries_per_wave in {1, 4, 16, 64} share a wave's walk,
+   LDS stack of stack_depth = 32 or 64 entries, 0 = the smallest that holds
+   the forest's depth; a stack shorter than the depth is never used) or
+   CKD_KNN_LANE (one walk per query; the default for sparser batches).
+   Results never depend on it. */
+int ckd_batch_config(ckd_batch *plan, int32_t knn_kernel, int32_t queries_per_wave, int32_t stack_depth);
 int ckd_batch_run(ckd_batch *plan, const void *queries, void *knn_dist, int64_t *knn_idx, int64_t *ball_off,
                   void *ball_dist, int64_t *ball_idx, void *stream);
 int ckd_batch_status(const ckd_batch *plan, const int64_t *ball_off, int64_t *ball_total, int32_t *overflow,

@@ -31,17 +31,20 @@ def workload(ckd):
     return sets, nq, ks, radii, qs
 
 
-def test_batch_matches_per_set_oracle(ckd, workload):
+@pytest.mark.parametrize("kernel,qpw,leaf", [("stack", 64, 16), ("stack", 16, 16), ("stack", 4, 8), ("stack", 1, 32),
+                                              ("lane", 64, 16)])
+def test_batch_matches_per_set_oracle(ckd, workload, kernel, qpw, leaf):
     from cosmokdtree_amd import batch
     sets, nq, ks, radii, qs = workload
-    forest = batch.build_forest(sets, leaf=16, boxsize=[L] * 3)
+    forest = batch.build_forest(sets, leaf=leaf, boxsize=[L] * 3)
     assert forest.info().nsets == len(sets)
-    plan = batch.BatchPlan(forest, nq, ks, radii, sorted=True, threads=128)
+    plan = batch.BatchPlan(forest, nq, ks, radii, sorted=True, threads=128, knn_kernel=kernel,
+                           queries_per_wave=qpw)
     out = plan.run(torch.from_numpy(np.concatenate(qs)).cuda())
     off = out["ball_off"].cpu().numpy()
     bd, bi = out["ball_dist"].cpu().numpy(), out["ball_idx"].cpu().numpy()
     for r, (pts, q) in enumerate(zip(sets, qs)):
-        orc = OracleTree(pts, 16, np.full(3, L))
+        orc = OracleTree(pts, leaf, np.full(3, L))
         k = int(ks[r])
         d_ref, i_ref = orc.knn(q, k, True)
         d_got, i_got = plan.knn_rows(r)

@@ -16,7 +16,9 @@ from tests.parity import assert_knn_rows
 
 pytestmark = pytest.mark.gpu
 
-KERNELS = [("lane", 0), ("wave", 0), ("stack", 32), ("stack", 64)]
+KERNELS = [("lane", 0, 0), ("wave", 0, 0), ("stack", 32, 0), ("stack", 64, 0), ("stack", 0, 16), ("stack", 0, 4),
+           ("stack", 0, 1)]
+IDS = [f"{k}{s or ''}{'_q%d' % w if w else ''}" for k, s, w in KERNELS]
 
 
 def _clustered(n, seed, dtype):
@@ -51,24 +53,24 @@ def kcase(request, ckd):
     orc.close()
 
 
-@pytest.mark.parametrize("kernel,stack", KERNELS, ids=[f"{k}{s or ''}" for k, s in KERNELS])
+@pytest.mark.parametrize("kernel,stack,qpw", KERNELS, ids=IDS)
 @pytest.mark.parametrize("k", [8, 32, 64])
-def test_kernel_matches_oracle(kcase, kernel, stack, k):
+def test_kernel_matches_oracle(kcase, kernel, stack, qpw, k):
     q = kcase["q"]
     d_ref, i_ref = kcase["orc"].knn(q, k, True)
-    tree = kcase["tree"].knn_config(kernel, stack)
+    tree = kcase["tree"].knn_config(kernel, stack, qpw)
     st = torch.zeros(6, dtype=torch.int64, device="cuda")
     d, i = tree.knn(torch.from_numpy(q), k, sorted=True, stats=st)
     assert_knn_rows(kcase["pts"], q, k, d_ref, i_ref, d.cpu().numpy(), i.cpu().numpy(), L=kcase["L"])
     assert int(st[3]) == len(q)
 
 
-@pytest.mark.parametrize("kernel,stack", KERNELS, ids=[f"{k}{s or ''}" for k, s in KERNELS])
-def test_kernel_seeded(kcase, kernel, stack):
+@pytest.mark.parametrize("kernel,stack,qpw", KERNELS, ids=IDS)
+def test_kernel_seeded(kcase, kernel, stack, qpw):
     """Seeds around the true k-th distance: about half are short and retried."""
     q = kcase["q"]
     k = 16
-    tree = kcase["tree"].knn_config(kernel, stack)
+    tree = kcase["tree"].knn_config(kernel, stack, qpw)
     d0, i0 = tree.knn(torch.from_numpy(q), k, sorted=True)
     rng = np.random.default_rng(4)
     seed = (d0[:, k - 1].double().cpu().numpy() * rng.uniform(0.5, 1.5, len(q))).astype(np.float32)
@@ -95,8 +97,8 @@ def test_deep_tree_falls_back(ckd):
         q = np.concatenate([pts, pts * 1.0001], axis=0)
         q = np.concatenate([q] * 3, axis=0)  # several waves, input order
         d_ref, i_ref = orc.knn(q, 8, True)
-        for kernel, stack in KERNELS:
-            d, i = tree.knn_config(kernel, stack).knn(torch.from_numpy(q), 8, sorted=True)
+        for kernel, stack, qpw in KERNELS:
+            d, i = tree.knn_config(kernel, stack, qpw).knn(torch.from_numpy(q), 8, sorted=True)
             assert_knn_rows(pts, q, 8, d_ref, i_ref, d.cpu().numpy(), i.cpu().numpy())
     finally:
         tree.free()
@@ -111,5 +113,7 @@ def test_knn_config_rejects(ckd):
             tree.knn_config("warp")
         with pytest.raises(Exception):
             tree.knn_config("stack", 48)
+        with pytest.raises(Exception):
+            tree.knn_config("stack", 0, 8)
     finally:
         tree.free()
