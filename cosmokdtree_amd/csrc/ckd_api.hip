@@ -76,8 +76,9 @@ static void release(ckd_tree *t) {
     if (t->idx) (void)hipFree(t->idx);
     if (t->wide) (void)hipFree(t->wide);
     if (t->wroot) (void)hipFree(t->wroot);
+    if (t->wrange) (void)hipFree(t->wrange);
     t->nodes = t->bbox = t->dbox = t->pts = t->idx = t->wide = nullptr;
-    t->wroot = nullptr;
+    t->wroot = t->wrange = nullptr;
     t->nwide = 0;
     t->range = nullptr;
 }

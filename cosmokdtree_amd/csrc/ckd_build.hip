@@ -523,11 +523,13 @@ __device__ __forceinline__ int2 child_pair(const NodeRec<T> *nodes, const Range 
 
 template <typename T, int D>
 __global__ void k_fill_wide(const NodeRec<T> *nodes, const T *dbox, const Range *range, const int64_t *widx,
-                            int64_t nn, char *wide) {
+                            int64_t nn, char *wide, int2 *wrange) {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= nn) return;
     const NodeRec<T> nr = nodes[v];
     if (nr.axis < 0) return;
+    const Range rv = range[v];
+    wrange[widx[v]] = make_int2((int32_t)rv.start, (int32_t)rv.count);
     char *base = wide + widx[v] * wide_bytes<T, D>();
     T *bx = (T *)base;
 #pragma unroll
@@ -647,9 +649,10 @@ int build_wide(ckd_tree *t, hipStream_t s) {
     CKD_HIP(hipStreamSynchronize(s));
     CKD_HIP(hipMalloc(&t->wide, wide_bytes<T, D>() * (size_t)(ninner > 0 ? ninner : 1)));
     CKD_HIP(hipMalloc((void **)&t->wroot, sizeof(int2) * (size_t)nsets));
+    CKD_HIP(hipMalloc((void **)&t->wrange, sizeof(int2) * (size_t)(ninner > 0 ? ninner : 1)));
     t->nwide = ninner;
     k_fill_wide<T, D><<<g, 256, 0, s>>>(nodes, (const T *)t->dbox, t->range, widx.as<int64_t>(), nn,
-                                        (char *)t->wide);
+                                        (char *)t->wide, t->wrange);
     k_root_pairs<T><<<(unsigned)ceil_div(nsets, 256), 256, 0, s>>>(nodes, t->range, widx.as<int64_t>(), nsets,
                                                                     t->wroot);
     CKD_HIP(hipGetLastError());

@@ -23,6 +23,8 @@
 //                   holding rows [a, a + b).  One load decides both children
 //                   and hands over a leaf's rows without touching nodes[].
 //   wroot[nsets]    the (a, b) pair of each set's root, same convention.
+//   wrange[nwide]   (first row, count) of each inner node, by wide id (the
+//                   ball walk takes a whole subtree without descending).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -73,6 +75,7 @@ struct TreeView {
     const void *idx;
     const char *wide;     // wide records (nullptr when not built)
     const int2 *wroot;    // root pair of each set
+    const int2 *wrange;   // rows of each inner node, by wide id
     int idx64;
     T L[CKD_MAX_DIM];  // box size per axis (periodic only)
 };
@@ -252,6 +255,7 @@ struct ckd_tree {
     void *wide = nullptr;    // wide records of the inner nodes
     int64_t nwide = 0;
     int2 *wroot = nullptr;   // device [nsets] root pairs
+    int2 *wrange = nullptr;  // device [nwide] inner-node rows
     int knn_kernel = -1;     // kNN kernel choice, -1 = automatic (ckd_knn_config)
     int knn_stack = 0;       // stack depth of the stack kernel, 0 = automatic
     int knn_qpw = 0;         // queries per wave of the stack kernel (1..64), 0 = 64
@@ -276,6 +280,7 @@ TreeView<T> view(const ckd_tree *t) {
     v.idx = t->idx;
     v.wide = (const char *)t->wide;
     v.wroot = t->wroot;
+    v.wrange = t->wrange;
     v.idx64 = t->idx_bits == 64;
     for (int i = 0; i < CKD_MAX_DIM; i++) v.L[i] = (T)t->L[i];
     return v;

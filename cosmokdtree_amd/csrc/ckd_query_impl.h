@@ -308,6 +308,28 @@ __global__ __launch_bounds__(QT) void k_knn_heap(TreeView<T> tv, const T *querie
 }
 
 // ---------------------------------------------------------------- ball / box
+// Ball search with one walk per wave (ball_stack), queries in leaf order.
+template <typename T, int D, bool PER, bool FILL, int SD>
+__global__ __launch_bounds__(QT) void k_ball_s(TreeView<T> tv, int2 root, int2 rootrows, const T *queries, int64_t nq,
+                                               const int32_t *order, T r2, int64_t *counts, const int64_t *offsets,
+                                               T *dout, int64_t *iout) {
+    __shared__ BallEntry stk[QT / WAVE][SD];
+    __shared__ EmitEntry eq[QT / WAVE][FILL ? EMIT_CAP : 1];
+    const int64_t blk = xcd_block();
+    const int64_t t = blk * QT + threadIdx.x;
+    if (blk * QT + (threadIdx.x & ~(WAVE - 1)) >= nq) return;  // whole wave idle
+    const bool active = t < nq;
+    const int64_t qi = active ? (order ? (int64_t)order[t] : t) : 0;
+    T q[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) q[d] = active ? queries[qi * D + d] : T(0);
+    const int64_t w0 = FILL && active ? offsets[qi] : 0;
+    const int wv = threadIdx.x / WAVE;
+    const int64_t c = ball_stack<T, D, PER, FILL, SD>(tv, root, tv.dbox, rootrows, q, active, r2, dout, iout, w0, 0,
+                                                      stk[wv], eq[wv]);
+    if (!FILL && active) counts[qi] = c;
+}
+
 template <typename T, int D, bool PER, bool FILL>
 __global__ __launch_bounds__(QT) void k_ball(TreeView<T> tv, const T *queries, int64_t nq, T r2, int64_t *counts,
                                              const int64_t *offsets, T *dout, int64_t *iout) {
@@ -560,12 +582,75 @@ int knn_launch(const ckd_tree *t, const T *q, int64_t nq, int k, int sorted, con
     return CKD_OK;
 }
 
+// Leaf order of the queries (k_locate + counting sort) into workspace slot 0.
+template <typename T, int D>
+int order_queries(const ckd_tree *t, const TreeView<T> &tv, const T *q, int64_t nq, const int32_t **order,
+                  const int32_t **leaf_of, hipStream_t s) {
+    const unsigned g = (unsigned)ceil_div(nq, QT);
+    const size_t a = 3 * sizeof(int32_t) * (size_t)nq, b = sizeof(int64_t) * (size_t)(t->nnodes + 1);
+    void *ws = nullptr;
+    int rc = workspace(const_cast<ckd_tree *>(t), 0, a + 2 * b + scan_tmp_bytes(t->nnodes) + 256, &ws, s);
+    if (rc) return rc;
+    int32_t *lo = (int32_t *)ws, *rk = lo + nq, *ord = rk + nq;
+    int64_t *cnt = (int64_t *)((char *)ws + ((a + 255) & ~(size_t)255));
+    int64_t *off = cnt + t->nnodes + 1;
+    void *scr = off + t->nnodes + 1;
+    k_fill_i64<<<(unsigned)ceil_div(t->nnodes, 256), 256, 0, s>>>(cnt, t->nnodes, 0);
+    k_locate<T, D><<<g, QT, 0, s>>>(tv, q, nq, lo, rk, cnt);
+    if ((rc = scan_i64(cnt, off, t->nnodes, false, scr, s))) return rc;
+    k_order<<<(unsigned)ceil_div(nq, 256), 256, 0, s>>>(lo, rk, off, nq, ord);
+    CKD_HIP(hipGetLastError());
+    *order = ord;
+    if (leaf_of) *leaf_of = lo;
+    return CKD_OK;
+}
+
+// CKD_BALL_KERNEL=lane|stack overrides the automatic choice of the ball
+// kernel (the walk per wave where the wide records allow it); read at every
+// call.
+int ball_kernel_env() {
+    const char *e = getenv("CKD_BALL_KERNEL");
+    return !e ? -1 : (e[0] == 'l' ? CKD_KNN_LANE : e[0] == 's' ? CKD_KNN_STACK : -1);
+}
+
 template <typename T, int D>
 int ball_launch(const ckd_tree *t, const T *q, int64_t nq, T r2, int64_t *counts, const int64_t *off, T *dist,
                 int64_t *idx, hipStream_t s) {
     const TreeView<T> tv = view<T>(t);
     const unsigned g = (unsigned)ceil_div(nq, QT);
     const bool fill = off != nullptr;
+    int kind = ball_kernel_env();
+    if (kind < 0) kind = CKD_KNN_STACK;
+    if constexpr (D <= 3) {
+        if (kind == CKD_KNN_STACK && t->wide && t->depth <= 64 && t->n < INT32_MAX) {
+            const int32_t *order = nullptr;
+            if (nq >= ORDER_MIN_QUERIES && nq < INT32_MAX && t->nnodes > 1) {
+                const int rc = order_queries<T, D>(t, tv, q, nq, &order, nullptr, s);
+                if (rc) return rc;
+            }
+            const int2 root = make_int2(0, t->nnodes > 1 ? -1 : (int32_t)t->n);
+            const int2 rows = make_int2(0, (int32_t)t->n);
+            if (t->depth <= 32) {
+                if (t->periodic) {
+                    if (fill) k_ball_s<T, D, true, true, 32><<<g, QT, 0, s>>>(tv, root, rows, q, nq, order, r2, counts, off, dist, idx);
+                    else k_ball_s<T, D, true, false, 32><<<g, QT, 0, s>>>(tv, root, rows, q, nq, order, r2, counts, off, dist, idx);
+                } else {
+                    if (fill) k_ball_s<T, D, false, true, 32><<<g, QT, 0, s>>>(tv, root, rows, q, nq, order, r2, counts, off, dist, idx);
+                    else k_ball_s<T, D, false, false, 32><<<g, QT, 0, s>>>(tv, root, rows, q, nq, order, r2, counts, off, dist, idx);
+                }
+            } else {
+                if (t->periodic) {
+                    if (fill) k_ball_s<T, D, true, true, 64><<<g, QT, 0, s>>>(tv, root, rows, q, nq, order, r2, counts, off, dist, idx);
+                    else k_ball_s<T, D, true, false, 64><<<g, QT, 0, s>>>(tv, root, rows, q, nq, order, r2, counts, off, dist, idx);
+                } else {
+                    if (fill) k_ball_s<T, D, false, true, 64><<<g, QT, 0, s>>>(tv, root, rows, q, nq, order, r2, counts, off, dist, idx);
+                    else k_ball_s<T, D, false, false, 64><<<g, QT, 0, s>>>(tv, root, rows, q, nq, order, r2, counts, off, dist, idx);
+                }
+            }
+            CKD_HIP(hipGetLastError());
+            return CKD_OK;
+        }
+    }
     if (t->periodic) {
         if (fill) k_ball<T, D, true, true><<<g, QT, 0, s>>>(tv, q, nq, r2, counts, off, dist, idx);
         else k_ball<T, D, true, false><<<g, QT, 0, s>>>(tv, q, nq, r2, counts, off, dist, idx);

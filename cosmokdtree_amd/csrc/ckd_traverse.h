@@ -740,6 +740,148 @@ __device__ __forceinline__ int64_t ball_one(const TreeView<T> &tv, const T (&q)[
     return cnt;
 }
 
+// ------------------------------------------------------------------
+// Ball search with one walk per wave (queries in leaf order), over the wide
+// records: the same decisions as ball_one for every lane -- a subtree whose
+// point box is farther than r is skipped, one entirely within r is taken
+// whole, a leaf otherwise tests d^2 <= r^2 -- but the subtrees are visited
+// once for all the lanes that touch them (the lanes that still need a
+// deferred subtree travel with it on the stack as a mask), and a subtree
+// taken whole is not copied by its lane: the lane queues (its rows, its
+// output offset) and the whole wave writes the queued rows together, 64
+// consecutive rows per step (coalesced loads and stores).
+struct BallEntry {  // a deferred subtree and the lanes that must visit it
+    int32_t a, b;
+    uint64_t mask;
+};
+struct EmitEntry {  // rows [start, start + count) for lane `lane`, out from `out`
+    int32_t start, count;
+    int64_t out;
+    int32_t lane, pad;
+};
+constexpr int EMIT_CAP = 2 * WAVE;  // two children per lane between drains
+
+template <typename T, int D, bool PER>
+__device__ __forceinline__ void drain_emits(const TreeView<T> &tv, const T (&q)[D], EmitEntry *eq, int &qn,
+                                            T *dout, int64_t *iout, int64_t idx_base) {
+    const int lane = threadIdx.x & (WAVE - 1);
+#pragma unroll 1
+    for (int e = 0; e < qn; e++) {
+        const EmitEntry E = eq[e];
+        const int owner = __builtin_amdgcn_readfirstlane(E.lane);
+        T qo[D];
+#pragma unroll
+        for (int d = 0; d < D; d++) qo[d] = __shfl(q[d], owner, WAVE);
+#pragma unroll 1
+        for (int32_t j = lane; j < E.count; j += WAVE) {
+            const int32_t p = E.start + j;
+            T x[D];
+            load_pt<T, D>(tv.pts, p, x);
+            dout[E.out + j] = sqrt_rn<T>(dist2<T, D, PER>(x, qo, tv.L));
+            iout[E.out + j] = load_idx(tv.idx, tv.idx64, p) - idx_base;
+        }
+    }
+    qn = 0;
+}
+
+// Returns this lane's count; with FILL writes its rows from dout/iout + w0.
+// rootbox/rootrows: the tight box and rows of the subtree at `root`.
+template <typename T, int D, bool PER, bool FILL, int SD>
+__device__ __forceinline__ int64_t ball_stack(const TreeView<T> &tv, int2 root, const T *rootbox, int2 rootrows,
+                                              const T (&q)[D], bool active, T r2, T *dout, int64_t *iout, int64_t w0,
+                                              int64_t idx_base, BallEntry *stk, EmitEntry *eq) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const uint64_t below = lane ? (~0ull >> (WAVE - lane)) : 0ull;  // lanes under this one
+    int64_t cnt = 0, w = w0;
+    int qn = 0;  // queued emits (wave-uniform)
+    // take rows [s, s + n) whole for the lanes in `in`
+    auto take = [&](bool in, int32_t s, int32_t n) {
+        const uint64_t m = __ballot(in);
+        if (in) {
+            cnt += n;
+            if (FILL) {
+                EmitEntry &E = eq[qn + __popcll(m & below)];
+                E.start = s;
+                E.count = n;
+                E.out = w;
+                E.lane = lane;
+                w += n;
+            }
+        }
+        if (FILL) qn += __popcll(m);
+    };
+    bool run = active;
+    {  // the root itself
+        T b[2 * D];
+#pragma unroll
+        for (int d = 0; d < 2 * D; d++) b[d] = rootbox[d];
+        const bool reach = run && !(mindist2<T, D, PER>(b, q, tv.L) > r2);
+        const bool whole = reach && maxdist2<T, D>(b, q) <= r2;
+        take(whole, rootrows.x, rootrows.y);
+        run = reach && !whole;
+        if (FILL && qn > 0) drain_emits<T, D, PER>(tv, q, eq, qn, dout, iout, idx_base);
+    }
+    int32_t a = root.x, b = root.y;
+    int sp = 0;
+    bool go = __any(run);
+    while (go) {
+        if (b >= 0) {  // a leaf some lanes only partly cover: test its rows
+#pragma unroll 1
+            for (int32_t p = a; p < a + b; p++) {
+                T x[D];
+                load_pt<T, D>(tv.pts, p, x);
+                const T d = dist2<T, D, PER>(x, q, tv.L);
+                if (run && d <= r2) {
+                    if (FILL) {
+                        dout[w] = sqrt_rn<T>(d);
+                        iout[w] = load_idx(tv.idx, tv.idx64, p) - idx_base;
+                        w++;
+                    }
+                    cnt++;
+                }
+            }
+        } else {
+            T b0[2 * D], b1[2 * D];
+            int4 r;
+            load_wide<T, D>(tv.wide, a, b0, b1, r);
+            const bool reach0 = run && !(mindist2<T, D, PER>(b0, q, tv.L) > r2);
+            const bool reach1 = run && !(mindist2<T, D, PER>(b1, q, tv.L) > r2);
+            const bool whole0 = reach0 && maxdist2<T, D>(b0, q) <= r2;
+            const bool whole1 = reach1 && maxdist2<T, D>(b1, q) <= r2;
+            if (__any(whole0)) {
+                const int2 rr = r.y >= 0 ? make_int2(r.x, r.y) : tv.wrange[r.x];
+                take(whole0, rr.x, rr.y);
+            }
+            if (__any(whole1)) {
+                const int2 rr = r.w >= 0 ? make_int2(r.z, r.w) : tv.wrange[r.z];
+                take(whole1, rr.x, rr.y);
+            }
+            if (FILL && qn > 0) drain_emits<T, D, PER>(tv, q, eq, qn, dout, iout, idx_base);
+            const bool t0 = reach0 && !whole0, t1 = reach1 && !whole1;
+            const uint64_t m0 = __ballot(t0), m1 = __ballot(t1);
+            if (m0 && m1) {  // left first, the right one deferred with its lanes
+                stk[sp].a = r.z;
+                stk[sp].b = r.w;
+                stk[sp].mask = m1;
+                sp++;
+            }
+            if (m0 || m1) {
+                a = __builtin_amdgcn_readfirstlane(m0 ? r.x : r.z);
+                b = __builtin_amdgcn_readfirstlane(m0 ? r.y : r.w);
+                run = m0 ? t0 : t1;
+                continue;
+            }
+        }
+        if (sp == 0) break;  // nothing deferred: done
+        sp--;
+        a = __builtin_amdgcn_readfirstlane(stk[sp].a);
+        b = __builtin_amdgcn_readfirstlane(stk[sp].b);
+        run = (stk[sp].mask >> lane) & 1ull;
+    }
+    if (FILL && qn > 0) drain_emits<T, D, PER>(tv, q, eq, qn, dout, iout, idx_base);
+    return cnt;
+}
+
 // Box search below `root` (box_search_recursive, cosmokdtree.f90:1258-1328):
 // visit the left child iff box_lo[axis] < split and the right child iff
 // box_hi[axis] > split; leaves test box_lo <= x <= box_hi on every axis.  A

@@ -32,7 +32,7 @@ def workload(ckd):
 
 
 @pytest.mark.parametrize("kernel,qpw,leaf", [("stack", 64, 16), ("stack", 16, 16), ("stack", 4, 8), ("stack", 1, 32),
-                                              ("lane", 64, 16)])
+                                              ("lane", 64, 16), ("lane", 64, 64)])
 def test_batch_matches_per_set_oracle(ckd, workload, kernel, qpw, leaf):
     from cosmokdtree_amd import batch
     sets, nq, ks, radii, qs = workload

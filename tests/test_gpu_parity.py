@@ -87,8 +87,10 @@ def _radius(case):
     return 0.3 if case["pts"].shape[1] == 6 else 0.02
 
 
+@pytest.mark.parametrize("kernel", ["stack", "lane"])
 @pytest.mark.parametrize("sorted_", [False, True])
-def test_ball(case, sorted_):
+def test_ball(case, sorted_, kernel, monkeypatch):
+    monkeypatch.setenv("CKD_BALL_KERNEL", kernel)  # walk per wave / one walk per query
     q = case["q"]
     r = _radius(case)
     off_r, d_r, i_r = case["orc"].ball_csr(q, r, sorted_)
