@@ -1,0 +1,174 @@
+// ckd_persist.hip -- a built tree saved to and loaded from a file, without
+// rebuilding: a fixed header, then every device array of the tree as it
+// lies in HBM.  No reference counterpart: the reference's tree is an opaque
+// pointer that cannot be serialised (src/cosmokdtree2py.f90:70-73).
+//
+// File layout (little-endian): PersistHeader, then the arrays in the order
+// of the table in save_arrays(), each as raw bytes.
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "ckd_internal.h"
+
+namespace ckd {
+namespace {
+
+constexpr char MAGIC[8] = {'C', 'K', 'D', 'T', 'R', 'E', 'E', '1'};
+
+struct PersistHeader {
+    char magic[8];
+    int64_t n, nnodes, nleaves, nsets, nwide;
+    int32_t ndim, prec, leafsize, periodic, idx_bits, depth;
+    double L[CKD_MAX_DIM];
+};
+
+struct Array {
+    void **ptr;
+    size_t bytes;
+};
+
+// The device arrays of a tree and their sizes.
+std::vector<Array> arrays(ckd_tree *t) {
+    const size_t T = (size_t)t->prec, nn = (size_t)t->nnodes, D = (size_t)t->ndim;
+    const size_t node_bytes = T == 4 ? 16 : 24;  // NodeRec<float> / NodeRec<double>
+    const size_t nw = (size_t)(t->nwide > 0 ? t->nwide : 1);
+    return {
+        {&t->nodes, node_bytes * nn},
+        {&t->bbox, T * 2 * D * nn},
+        {&t->dbox, T * 2 * D * nn},
+        {(void **)&t->range, sizeof(Range) * nn},
+        {&t->pts, T * (size_t)pstride((int)D) * (size_t)(t->n + PTS_PAD)},
+        {&t->idx, (size_t)(t->idx_bits / 8) * (size_t)t->n},
+        {&t->wide, t->wide ? wide_bytes_rt((int)D, (int)T) * nw : 0},
+        {(void **)&t->wroot, t->wroot ? sizeof(int2) * (size_t)t->nsets : 0},
+        {(void **)&t->wrange, t->wrange ? sizeof(int2) * nw : 0},
+        {(void **)&t->set_off, t->set_off ? sizeof(int64_t) * (size_t)(t->nsets + 1) : 0},
+    };
+}
+
+}  // namespace
+}  // namespace ckd
+
+using namespace ckd;
+
+extern "C" {
+
+int ckd_tree_save(const ckd_tree *tc, const char *path) {
+    if (!tc || !path) {
+        set_error("tree and path must not be NULL");
+        return CKD_ERR_ARG;
+    }
+    ckd_tree *t = const_cast<ckd_tree *>(tc);
+    CKD_HIP(hipSetDevice(t->device));
+    CKD_HIP(hipDeviceSynchronize());
+    FILE *f = fopen(path, "wb");
+    if (!f) {
+        set_error("cannot open %s for writing", path);
+        return CKD_ERR_ARG;
+    }
+    PersistHeader h;
+    memset(&h, 0, sizeof(h));
+    memcpy(h.magic, MAGIC, 8);
+    h.n = t->n;
+    h.nnodes = t->nnodes;
+    h.nleaves = t->nleaves;
+    h.nsets = t->nsets;
+    h.nwide = t->wide ? t->nwide : -1;
+    h.ndim = t->ndim;
+    h.prec = t->prec;
+    h.leafsize = t->leafsize;
+    h.periodic = t->periodic;
+    h.idx_bits = t->idx_bits;
+    h.depth = t->depth;
+    for (int d = 0; d < CKD_MAX_DIM; d++) h.L[d] = t->L[d];
+    bool ok = fwrite(&h, sizeof(h), 1, f) == 1;
+    std::vector<char> host;
+    for (const Array &a : arrays(t)) {
+        if (!ok || a.bytes == 0) continue;
+        host.resize(a.bytes);
+        hipError_t e = hipMemcpy(host.data(), *a.ptr, a.bytes, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) {
+            fclose(f);
+            return hip_fail(e, "hipMemcpy", __FILE__, __LINE__);
+        }
+        ok = fwrite(host.data(), 1, a.bytes, f) == a.bytes;
+    }
+    ok = (fclose(f) == 0) && ok;
+    if (!ok) {
+        set_error("writing %s failed", path);
+        return CKD_ERR_ARG;
+    }
+    return CKD_OK;
+}
+
+int ckd_tree_load(const char *path, void *stream, ckd_tree **out) {
+    if (!path || !out) {
+        set_error("path and out must not be NULL");
+        return CKD_ERR_ARG;
+    }
+    *out = nullptr;
+    FILE *f = fopen(path, "rb");
+    if (!f) {
+        set_error("cannot open %s", path);
+        return CKD_ERR_ARG;
+    }
+    PersistHeader h;
+    if (fread(&h, sizeof(h), 1, f) != 1 || memcmp(h.magic, MAGIC, 8) != 0 || h.n < 1 || h.ndim < 1 ||
+        h.ndim > CKD_MAX_DIM || (h.prec != 4 && h.prec != 8) || (h.idx_bits != 32 && h.idx_bits != 64) ||
+        h.nnodes < 1 || h.nsets < 1) {
+        fclose(f);
+        set_error("%s is not a saved cosmokdtree_amd tree", path);
+        return CKD_ERR_ARG;
+    }
+    ckd_tree *t = new ckd_tree();
+    t->n = h.n;
+    t->nnodes = h.nnodes;
+    t->nleaves = h.nleaves;
+    t->nsets = h.nsets;
+    t->ndim = h.ndim;
+    t->prec = h.prec;
+    t->leafsize = h.leafsize;
+    t->periodic = h.periodic;
+    t->idx_bits = h.idx_bits;
+    t->depth = h.depth;
+    for (int d = 0; d < CKD_MAX_DIM; d++) t->L[d] = h.L[d];
+    t->nwide = h.nwide >= 0 ? h.nwide : 0;
+    (void)hipGetDevice(&t->device);
+    // allocate what the header says the file holds (the wide records only if saved)
+    const bool has_wide = h.nwide >= 0;
+    const size_t nw = (size_t)(t->nwide > 0 ? t->nwide : 1);
+    std::vector<Array> as = arrays(t);
+    // sizes of the optional arrays follow presence, as at save time
+    as[6].bytes = has_wide ? wide_bytes_rt(t->ndim, t->prec) * nw : 0;
+    as[7].bytes = has_wide ? sizeof(int2) * (size_t)t->nsets : 0;
+    as[8].bytes = has_wide ? sizeof(int2) * nw : 0;
+    as[9].bytes = sizeof(int64_t) * (size_t)(t->nsets + 1);  // always saved
+    std::vector<char> host;
+    int rc = CKD_OK;
+    for (Array &a : as) {
+        if (a.bytes == 0) continue;
+        host.resize(a.bytes);
+        if (fread(host.data(), 1, a.bytes, f) != a.bytes) {
+            set_error("%s is truncated", path);
+            rc = CKD_ERR_ARG;
+            break;
+        }
+        hipError_t e = hipMalloc(a.ptr, a.bytes);
+        if (e == hipSuccess) e = hipMemcpyAsync(*a.ptr, host.data(), a.bytes, hipMemcpyHostToDevice, (hipStream_t)stream);
+        if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
+        if (e != hipSuccess) {
+            rc = hip_fail(e, "loading a tree", __FILE__, __LINE__);
+            break;
+        }
+    }
+    fclose(f);
+    if (rc != CKD_OK) {
+        ckd_free(t);
+        return rc;
+    }
+    *out = t;
+    return CKD_OK;
+}
+
+}  // extern "C"

@@ -102,6 +102,11 @@ class KDTree(int):
               "knn_config")
         return self
 
+    def save(self, path: str):
+        """Write the built tree to `path` (reload with kdtree.load_tree)."""
+        self._check_alive()
+        check(lib.ckd_tree_save(self._ptr, str(path).encode()), "save")
+
     def knn_timing(self, on: bool = True):
         """Start (on=True) or stop recording HIP events around this tree's kNN
         traversal kernel launches; returns (summed ms, launches) of the
@@ -221,6 +226,23 @@ def _build(points, leaf: int, boxsize, keep_points: bool = False) -> KDTree:
                   periodic=boxsize is not None,
                   boxsize=None if boxsize is None else np.asarray(boxsize, dtype=np.float64),
                   points=pts if keep_points else None)
+    _REGISTRY[int(tree)] = tree
+    return tree
+
+
+def load_tree(path: str) -> KDTree:
+    """A tree saved by KDTree.save, on the current device, ready to query."""
+    _require_gpu()
+    out = C.c_void_p()
+    check(lib.ckd_tree_load(str(path).encode(), _stream(), C.byref(out)), "load")
+    inf = TreeInfo()
+    check(lib.ckd_get_info(out, C.byref(inf)), "ckd_get_info")
+    prec, d = inf.prec, inf.ndim
+    dt = torch.float32 if prec == 4 else torch.float64
+    bs = np.array(inf.boxsize[:d], dtype=np.float64) if inf.periodic else None
+    tree = KDTree(out.value, n=inf.n, ndim=d, leafsize=inf.leafsize, torch_dtype=dt,
+                  np_dtype=np.float32 if prec == 4 else np.float64, device=torch.device("cuda", torch.cuda.current_device()),
+                  periodic=bool(inf.periodic), boxsize=bs, points=None)
     _REGISTRY[int(tree)] = tree
     return tree
 

@@ -194,6 +194,16 @@ int ckd_tree_dump(const ckd_tree *tree, int32_t *axis, void *split, void *lo, vo
                   int32_t *parent, int64_t *start, int64_t *count, void *points, int64_t *idx);
 
 /*
+ * Save a built tree (or forest) to a file [host path] and load it back on
+ * the current device without rebuilding: a fixed header, then every device
+ * array as it lies in HBM.  A loaded tree answers every query exactly as
+ * the saved one.  No reference counterpart: the reference's tree handle is
+ * an opaque pointer (cosmokdtree2py.f90:70-73) that cannot be persisted.
+ */
+int ckd_tree_save(const ckd_tree *tree, const char *path);
+int ckd_tree_load(const char *path, void *stream, ckd_tree **out);
+
+/*
  * Batched queries over a forest (ckd_build_forest), drained by persistent
  * kernels from a device work queue.  No reference counterpart (the
  * reference answers one tree per call); each set's results equal the

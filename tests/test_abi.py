@@ -52,3 +52,18 @@ def test_package_imports_and_validates_arguments():
         kdtree.knn_search(12345, np.zeros(3), 3.0)
     with pytest.raises(AssertionError):
         kdtree.ball_search(12345, np.zeros(3), -1.0)
+
+
+def test_tree_load_rejects_other_files(tmp_path):
+    """A file that is not a saved tree is refused before any device call."""
+    import ctypes
+    path = os.path.join(ROOT, "cosmokdtree_amd", "libcosmokdtree_amd.so")
+    lib = ctypes.CDLL(path)
+    lib.ckd_tree_load.argtypes = [ctypes.c_char_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]
+    lib.ckd_last_error.restype = ctypes.c_char_p
+    bad = tmp_path / "not_a_tree.bin"
+    bad.write_bytes(b"definitely not a tree" * 10)
+    out = ctypes.c_void_p()
+    assert lib.ckd_tree_load(str(bad).encode(), None, ctypes.byref(out)) == -1
+    assert b"not a saved" in lib.ckd_last_error()
+    assert lib.ckd_tree_load(str(tmp_path / "missing").encode(), None, ctypes.byref(out)) == -1
