@@ -216,6 +216,26 @@ def main():
     extra["build_roofline_frac"] = extra["build_bytes_per_point"] * N / (build_ms / 1e3) / 1e9 / HBM_PEAK_GBS
 
     if not args.no_extra:
+        # SPH smoothing length + density of every point from its k neighbours,
+        # fused into the traversal (writes 8 B per query instead of the k-row)
+        h, rho = tree.sph_density(P, K)
+        del h, rho
+        torch.cuda.synchronize()
+        tree.knn_timing(True)
+        e0.record()
+        for _ in range(args.steps):
+            h, rho = tree.sph_density(P, K)
+        e1.record()
+        torch.cuda.synchronize()
+        sms, sn = tree.knn_timing(False)
+        sph_ms = e0.elapsed_time(e1) / args.steps
+        extra["sph_density_qps"] = N / (sph_ms / 1e3)
+        extra["sph_density_ms_per_step"] = sph_ms
+        extra["sph_density_kernel_ms"] = sms / max(sn, 1)
+        extra["sph_density_mean_rho"] = float(rho.double().mean().item())
+        log(f"SPH density k={K}: {sph_ms:.2f} ms/step ({N / sph_ms / 1e3:.1f} Mq/s), "
+            f"kernel {extra['sph_density_kernel_ms']:.2f} ms")
+        del h, rho
         # ball search r = 0.2 on a random 1M subset (float32 tree)
         rng = np.random.Generator(np.random.PCG64(SEED_SUBSET))
         sub = torch.from_numpy(pts[rng.choice(N, 1 << 20, replace=False)]).to(dev)

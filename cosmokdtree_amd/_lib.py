@@ -27,7 +27,7 @@ CKD_MAX_DIM = 8
 EXPORTS = (
     "ckd_build", "ckd_build_forest", "ckd_free", "ckd_get_info", "ckd_knn", "ckd_ball_count", "ckd_ball_fill",
     "ckd_box_count", "ckd_box_fill", "ckd_tree_dump", "ckd_last_error", "ckd_version", "ckd_knn_config",
-    "ckd_knn_timing", "ckd_knn_learned", "ckd_mlp_table", "ckd_tree_save", "ckd_tree_load",
+    "ckd_knn_timing", "ckd_knn_learned", "ckd_mlp_table", "ckd_tree_save", "ckd_tree_load", "ckd_sph_density",
 )
 KNN_KERNELS = {"auto": -1, "lane": 0, "wave": 1, "stack": 2}
 
@@ -53,6 +53,7 @@ def _load():
     lib.ckd_get_info.argtypes = [vp, C.POINTER(TreeInfo)]
     lib.ckd_knn.argtypes = [vp, vp, i64, i32, i32, vp, vp, vp, vp, vp]
     lib.ckd_knn_config.argtypes = [vp, i32, i32, i32]
+    lib.ckd_sph_density.argtypes = [vp, vp, i64, i32, vp, vp, vp, vp]
     lib.ckd_knn_timing.argtypes = [vp, i32, C.POINTER(C.c_double), C.POINTER(i64)]
     lib.ckd_tree_save.argtypes = [vp, C.c_char_p]
     lib.ckd_tree_load.argtypes = [C.c_char_p, vp, C.POINTER(vp)]

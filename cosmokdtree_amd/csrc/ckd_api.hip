@@ -12,6 +12,8 @@ int query_knn(const ckd_tree *t, const void *q, int64_t nq, int k, int sorted, c
 int query_knn_learned(const ckd_tree *t, const void *q, int64_t nq, int k, int sorted, const float *params,
                       float margin, int G, const uint16_t *table, void *dist, int64_t *idx, int64_t *stats,
                       hipStream_t s);
+int query_sph(const ckd_tree *t, const void *q, int64_t nq, int k, const void *mass, void *h, void *rho,
+              hipStream_t s);
 int query_ball(const ckd_tree *t, const void *q, int64_t nq, double r, int64_t *counts, const int64_t *off,
                void *dist, int64_t *idx, hipStream_t s);
 int query_box(const ckd_tree *t, const void *b, int64_t nb, int64_t *counts, const int64_t *off, int64_t *idx,
@@ -291,6 +293,29 @@ int ckd_knn_learned(const ckd_tree *t, const void *queries, int64_t nq, int32_t 
     if (nq == 0) return CKD_OK;
     return query_knn_learned(t, queries, nq, k, sorted, params, margin, G, table, dist, idx, stats,
                              (hipStream_t)stream);
+}
+
+int ckd_sph_density(const ckd_tree *t, const void *queries, int64_t nq, int32_t k, const void *mass, void *h,
+                    void *rho, void *stream) {
+    g_err[0] = 0;
+    if (!t) {
+        set_error("tree must not be NULL");
+        return CKD_ERR_ARG;
+    }
+    if (t->ndim != 3) {
+        set_error("the SPH density step needs a 3-D tree, got ndim = %d", t->ndim);
+        return CKD_ERR_ARG;
+    }
+    if (k < 2 || k > 64 || k > t->n) {
+        set_error("the SPH density step needs 2 <= k <= min(64, n), got k = %d", k);
+        return CKD_ERR_ARG;
+    }
+    if (nq < 0 || (nq > 0 && (!queries || !h || !rho))) {
+        set_error("invalid query or output arrays");
+        return CKD_ERR_ARG;
+    }
+    if (nq == 0) return CKD_OK;
+    return query_sph(t, queries, nq, k, mass, h, rho, (hipStream_t)stream);
 }
 
 int ckd_knn_config(ckd_tree *t, int32_t kernel, int32_t stack_depth, int32_t queries_per_wave) {

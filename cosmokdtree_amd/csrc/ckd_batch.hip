@@ -133,14 +133,17 @@ __device__ __forceinline__ void batch_knn_stack(const TreeView<T> &tv, const Bat
     }
 }
 
-template <typename T, int D, bool PER, int NT, int SD>
+// GC = 1 compiles the one-lane-per-query stack walk with G folded away (as
+// k_knn_s does); GC = 0 takes G from the argument.
+template <typename T, int D, bool PER, int NT, int SD, int GC>
 __global__ __launch_bounds__(NT, 3) void k_batch(TreeView<T> tv, const int64_t *set_off, const T *queries,
                                               const BatchItem *items, int64_t nitems, int64_t *head,
                                               const int32_t *kset, const double *rset, const int64_t *qoff,
                                               const int64_t *knn_off, int sorted, T *kdist, int64_t *kidx,
                                               int64_t *ball_counts, const int64_t *ball_off, T *bdist,
                                               int64_t *bidx, int64_t capacity, int64_t *overflow,
-                                              const int32_t *order, int G) {
+                                              const int32_t *order, int Grt) {
+    const int G = GC ? GC : Grt;
     __shared__ int64_t s_item;
     constexpr int CBMAX = sizeof(T) == 4 ? 16 : 8;
     __shared__ T buf_d[CBMAX * NT];
@@ -223,20 +226,28 @@ int run_t(ckd_batch *b, const T *queries, T *kdist, int64_t *kidx, int64_t *ball
         order = b->order;
     }
     const int G = WAVE / b->qpw;
-    if (b->n_a > 0)
-        k_batch<T, D, PER, NT, SD><<<b->grid, NT, 0, s>>>(tv, t->set_off, queries, b->items_a, b->n_a, b->counters,
-                                                          b->kset, b->rset, b->qoff, b->knn_off, b->sorted, kdist,
-                                                          kidx, b->counts, ball_off, bdist, bidx, b->capacity,
-                                                          b->counters + 2, order, G);
+    if (b->n_a > 0) {
+        if (G == 1 || !order)
+            k_batch<T, D, PER, NT, SD, 1><<<b->grid, NT, 0, s>>>(tv, t->set_off, queries, b->items_a, b->n_a,
+                                                                 b->counters, b->kset, b->rset, b->qoff, b->knn_off,
+                                                                 b->sorted, kdist, kidx, b->counts, ball_off, bdist,
+                                                                 bidx, b->capacity, b->counters + 2, order, 1);
+        else
+            k_batch<T, D, PER, NT, SD, 0><<<b->grid, NT, 0, s>>>(tv, t->set_off, queries, b->items_a, b->n_a,
+                                                                 b->counters, b->kset, b->rset, b->qoff, b->knn_off,
+                                                                 b->sorted, kdist, kidx, b->counts, ball_off, bdist,
+                                                                 bidx, b->capacity, b->counters + 2, order, G);
+    }
     CKD_HIP(hipGetLastError());
     if (b->has_ball) {
         int rc = scan_i64(b->counts, ball_off, b->nq, true, b->scan_tmp, s);
         if (rc) return rc;
         if (b->n_b > 0)
-            k_batch<T, D, PER, NT, SD><<<b->grid, NT, 0, s>>>(tv, t->set_off, queries, b->items_b, b->n_b,
-                                                              b->counters + 1, b->kset, b->rset, b->qoff, b->knn_off,
-                                                              b->sorted, kdist, kidx, b->counts, ball_off, bdist,
-                                                              bidx, b->capacity, b->counters + 2, nullptr, G);
+            k_batch<T, D, PER, NT, SD, 1><<<b->grid, NT, 0, s>>>(tv, t->set_off, queries, b->items_b, b->n_b,
+                                                                 b->counters + 1, b->kset, b->rset, b->qoff,
+                                                                 b->knn_off, b->sorted, kdist, kidx, b->counts,
+                                                                 ball_off, bdist, bidx, b->capacity, b->counters + 2,
+                                                                 nullptr, 1);
         CKD_HIP(hipGetLastError());
     }
     return CKD_OK;

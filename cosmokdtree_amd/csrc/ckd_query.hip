@@ -16,6 +16,8 @@ int query_ball_f32(const ckd_tree *, const void *, int64_t, double, int64_t *, c
 int query_ball_f64(const ckd_tree *, const void *, int64_t, double, int64_t *, const int64_t *, void *, int64_t *,
                    hipStream_t);
 int query_box_f32(const ckd_tree *, const void *, int64_t, int64_t *, const int64_t *, int64_t *, hipStream_t);
+int query_sph_f32(const ckd_tree *, const void *, int64_t, int, const void *, void *, void *, hipStream_t);
+int query_sph_f64(const ckd_tree *, const void *, int64_t, int, const void *, void *, void *, hipStream_t);
 int query_box_f64(const ckd_tree *, const void *, int64_t, int64_t *, const int64_t *, int64_t *, hipStream_t);
 
 int query_knn(const ckd_tree *t, const void *q, int64_t nq, int k, int sorted, const float *seed, void *dist,
@@ -35,6 +37,11 @@ int query_ball(const ckd_tree *t, const void *q, int64_t nq, double r, int64_t *
                void *dist, int64_t *idx, hipStream_t s) {
     return t->prec == 4 ? query_ball_f32(t, q, nq, r, counts, off, dist, idx, s)
                         : query_ball_f64(t, q, nq, r, counts, off, dist, idx, s);
+}
+
+int query_sph(const ckd_tree *t, const void *q, int64_t nq, int k, const void *mass, void *h, void *rho,
+              hipStream_t s) {
+    return t->prec == 4 ? query_sph_f32(t, q, nq, k, mass, h, rho, s) : query_sph_f64(t, q, nq, k, mass, h, rho, s);
 }
 
 int query_box(const ckd_tree *t, const void *b, int64_t nb, int64_t *counts, const int64_t *off, int64_t *idx,

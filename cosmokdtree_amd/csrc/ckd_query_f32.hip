@@ -20,6 +20,12 @@ int query_ball_f32(const ckd_tree *t, const void *q, int64_t nq, double r, int64
     return ball_t<float>(t, q, nq, r, counts, off, dist, idx, s);
 }
 
+int query_sph_f32(const ckd_tree *t, const void *q, int64_t nq, int k, const void *mass, void *h, void *rho,
+                   hipStream_t s) {
+    return t->periodic ? sph_launch<float, true>(t, (const float *)q, nq, k, (const float *)mass, (float *)h, (float *)rho, s)
+                       : sph_launch<float, false>(t, (const float *)q, nq, k, (const float *)mass, (float *)h, (float *)rho, s);
+}
+
 int query_box_f32(const ckd_tree *t, const void *b, int64_t nb, int64_t *counts, const int64_t *off, int64_t *idx,
                   hipStream_t s) {
     return box_t<float>(t, b, nb, counts, off, idx, s);

@@ -20,6 +20,12 @@ int query_ball_f64(const ckd_tree *t, const void *q, int64_t nq, double r, int64
     return ball_t<double>(t, q, nq, r, counts, off, dist, idx, s);
 }
 
+int query_sph_f64(const ckd_tree *t, const void *q, int64_t nq, int k, const void *mass, void *h, void *rho,
+                   hipStream_t s) {
+    return t->periodic ? sph_launch<double, true>(t, (const double *)q, nq, k, (const double *)mass, (double *)h, (double *)rho, s)
+                       : sph_launch<double, false>(t, (const double *)q, nq, k, (const double *)mass, (double *)h, (double *)rho, s);
+}
+
 int query_box_f64(const ckd_tree *t, const void *b, int64_t nb, int64_t *counts, const int64_t *off, int64_t *idx,
                   hipStream_t s) {
     return box_t<double>(t, b, nb, counts, off, idx, s);

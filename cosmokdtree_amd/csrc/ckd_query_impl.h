@@ -174,12 +174,15 @@ __global__ __launch_bounds__(QT, 3) void k_knn_w(TreeView<T> tv, const T *querie
 // cheaper than the LDS round trip and the occupancy the buffer costs); 8
 // slots per lane in double precision keep the LDS within 4 blocks per CU.
 // G lanes per query (1, 2, 4, ..., 64; 64 / G queries per wave): the grid
-// covers nq * G lanes.
-template <typename T, int D, bool PER, int KMAX, int SD>
+// covers nq * G lanes.  GC = 1 compiles the one-lane-per-query walk with G
+// folded away (the group scans cost registers and time even when G is 1 at
+// run time); GC = 0 takes G from the argument.
+template <typename T, int D, bool PER, int KMAX, int SD, int GC>
 __global__ __launch_bounds__(QT, 3) void k_knn_s(TreeView<T> tv, int2 root, const T *queries, int64_t nq,
                                               const int64_t *nq_dev, int k, int sorted, const float *seed,
                                               const int32_t *order, T *dout, int64_t *iout, int64_t *stats,
-                                              Retry rt, int G) {
+                                              Retry rt, int Grt) {
+    const int G = GC ? GC : Grt;
     constexpr int CB = KMAX < 16 ? 0 : (sizeof(T) == 4 ? 16 : 8);
     __shared__ T buf_d[CB > 0 ? CB * QT : 1];
     __shared__ int32_t buf_s[CB > 0 ? CB * QT : 1];
@@ -210,6 +213,31 @@ __global__ __launch_bounds__(QT, 3) void k_knn_s(TreeView<T> tv, int2 root, cons
         if (!active || sub != 0) c = Counters();
         flush_stats(c, active && sub == 0 && !nq_dev ? 1 : 0, stats);
     }
+}
+
+// SPH smoothing length and density of each query, fused after the stack
+// walk (sph_row): writes h[qi], rho[qi] instead of the k-row.
+template <typename T, bool PER, int KMAX, int SD>
+__global__ __launch_bounds__(QT, 3) void k_sph_s(TreeView<T> tv, int2 root, const T *queries, int64_t nq, int k,
+                                              const int32_t *order, const T *mass, T *hout, T *rhoout) {
+    constexpr int D = 3;
+    constexpr int CB = KMAX < 16 ? 0 : (sizeof(T) == 4 ? 16 : 8);
+    __shared__ T buf_d[CB > 0 ? CB * QT : 1];
+    __shared__ int32_t buf_s[CB > 0 ? CB * QT : 1];
+    __shared__ StackEntry<T, D> stk[QT / WAVE][SD];
+    const int64_t blk = xcd_block();
+    const int64_t t = blk * QT + threadIdx.x;
+    if (blk * QT + (threadIdx.x & ~(WAVE - 1)) >= nq) return;  // whole wave idle
+    Counters c;
+    const bool active = t < nq;
+    const int64_t qi = active ? (order ? (int64_t)order[t] : t) : 0;
+    T q[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) q[d] = active ? queries[qi * D + d] : T(0);
+    int found = 0;
+    knn_stack<T, D, PER, KMAX, SD, CB, 1>(tv, root, q, active, k, 1, T(FLT_MAX), hout + qi, nullptr, 0, c, buf_d,
+                                         buf_s, threadIdx.x, QT, stk[threadIdx.x / WAVE], found, 1, 0, 0, mass,
+                                         rhoout + qi);
 }
 
 // Positions flagged by round one, compacted in order: list[off[t]] = query.
@@ -393,12 +421,17 @@ int launch_wave(const ckd_tree *t, const TreeView<T> &tv, int kind, int sd, int 
             const int2 root = make_int2(0, t->nnodes > 1 ? -1 : (int32_t)t->n);
             const int G = WAVE / qpw;
             const unsigned gs = (unsigned)ceil_div(nq * G, QT);
-            if (sd <= 32)
-                k_knn_s<T, D, PER, KMAX, 32><<<gs, QT, 0, s>>>(tv, root, q, nq, nq_dev, k, sorted, seed, order, dist,
-                                                                idx, stats, rt, G);
-            else
-                k_knn_s<T, D, PER, KMAX, 64><<<gs, QT, 0, s>>>(tv, root, q, nq, nq_dev, k, sorted, seed, order, dist,
-                                                                idx, stats, rt, G);
+#define CKD_KNN_S(SD_, GC_)                                                                                  \
+    k_knn_s<T, D, PER, KMAX, SD_, GC_><<<gs, QT, 0, s>>>(tv, root, q, nq, nq_dev, k, sorted, seed, order, dist, idx, \
+                                                         stats, rt, G)
+            if (sd <= 32) {
+                if (G == 1) CKD_KNN_S(32, 1);
+                else CKD_KNN_S(32, 0);
+            } else {
+                if (G == 1) CKD_KNN_S(64, 1);
+                else CKD_KNN_S(64, 0);
+            }
+#undef CKD_KNN_S
         }
     } else if constexpr (D <= 3) {
         k_knn_w<T, D, PER, KMAX><<<g, QT, 0, s>>>(tv, q, nq, nq_dev, k, sorted, seed, order, dist, idx, stats, rt);
@@ -603,6 +636,37 @@ int order_queries(const ckd_tree *t, const TreeView<T> &tv, const T *q, int64_t 
     *order = ord;
     if (leaf_of) *leaf_of = lo;
     return CKD_OK;
+}
+
+template <typename T, bool PER>
+int sph_launch(const ckd_tree *t, const T *q, int64_t nq, int k, const T *mass, T *h, T *rho, hipStream_t s) {
+    const TreeView<T> tv = view<T>(t);
+    if (!stack_ok<T, 3>(t, 64) || t->n >= INT32_MAX) {
+        set_error("the SPH step needs a 3-D tree of depth <= 64 and fewer than 2^31 points");
+        return CKD_ERR_LIMIT;
+    }
+    const int32_t *order = nullptr;
+    if (nq >= ORDER_MIN_QUERIES && nq < INT32_MAX && t->nnodes > 1) {
+        const int rc = order_queries<T, 3>(t, tv, q, nq, &order, nullptr, s);
+        if (rc) return rc;
+    }
+    const int2 root = make_int2(0, t->nnodes > 1 ? -1 : (int32_t)t->n);
+    const unsigned g = (unsigned)ceil_div(nq, QT);
+    LaunchTimer tm{t};
+    int rc = tm.start(s);
+    if (rc) return rc;
+#define CKD_SPH(KM)                                                                                        \
+    do {                                                                                                  \
+        if (t->depth <= 32) k_sph_s<T, PER, KM, 32><<<g, QT, 0, s>>>(tv, root, q, nq, k, order, mass, h, rho); \
+        else k_sph_s<T, PER, KM, 64><<<g, QT, 0, s>>>(tv, root, q, nq, k, order, mass, h, rho);              \
+    } while (0)
+    if (k <= 8) CKD_SPH(8);
+    else if (k <= 16) CKD_SPH(16);
+    else if (k <= 32) CKD_SPH(32);
+    else CKD_SPH(64);
+#undef CKD_SPH
+    CKD_HIP(hipGetLastError());
+    return tm.stop(s);
 }
 
 // CKD_BALL_KERNEL=lane|stack overrides the automatic choice of the ball

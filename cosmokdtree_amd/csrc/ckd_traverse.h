@@ -628,17 +628,48 @@ __device__ __forceinline__ bool stack_walk(const TreeView<T> &tv, const T (&q)[D
     }
 }
 
+// SPH density of a query from its k nearest neighbours (the step that
+// follows kNN in SPH codes): smoothing length h = r_k / 2, so the cubic
+// spline kernel's support 2h reaches the k-th neighbour, and
+// rho = sum_j m_j W(r_j, h), W(r, h) = 1/(pi h^3) w(r/h),
+// w(q) = 1 - 1.5 q^2 + 0.75 q^3 (q < 1), 0.25 (2 - q)^3 (q < 2), else 0
+// (Monaghan & Lattanzio 1985).  The query itself counts when it is a tree
+// point (r = 0).  mass: per original point, nullptr = unit masses.
+template <typename T, int KMAX>
+__device__ __forceinline__ void sph_row(const TreeView<T> &tv, const T (&bd)[KMAX], const int32_t (&bs)[KMAX], int k,
+                                        const T *mass, int64_t idx_base, T *h_out, T *rho_out) {
+    const T h = sqrt_rn<T>(bd[KMAX - 1]) * T(0.5);
+    const T inv_h = T(1) / h;
+    T rho = T(0);
+#pragma unroll
+    for (int j = 0; j < KMAX; j++) {
+        if (j >= KMAX - k && bs[j] >= 0) {
+            const T q = sqrt_rn<T>(bd[j]) * inv_h;
+            const T tq = T(2) - q;
+            const T w = q < T(1) ? T(1) - T(1.5) * q * q + T(0.75) * q * q * q
+                                 : (q < T(2) ? T(0.25) * tq * tq * tq : T(0));
+            const T m = mass ? mass[load_idx(tv.idx, tv.idx64, bs[j]) - idx_base] : T(1);
+            rho = rho + m * w;
+        }
+    }
+    *h_out = h;
+    *rho_out = rho * inv_h * inv_h * inv_h * T(0.31830988618379067154);  // 1/pi
+}
+
 // Returns false for a seeded lane whose seed held fewer than k points (it
 // writes nothing then and sets `found` to the number it held; the caller
 // searches it again with a larger or no seed).  `stk` is this wave's stack
 // of SD entries in LDS.  With G > 1 each query is held by G lanes (lane
 // gbase + sub of its group; 64 / G queries per wave): they split the leaf
-// rows, keep identical lists, and lane sub == 0 writes the row.
-template <typename T, int D, bool PER, int KMAX, int SD, int CB>
+// rows, keep identical lists, and lane sub == 0 writes the row.  EPI = 1
+// writes the SPH smoothing length to *drow and density to *rho_out instead
+// of the row (sph_row).
+template <typename T, int D, bool PER, int KMAX, int SD, int CB, int EPI = 0>
 __device__ __forceinline__ bool knn_stack(const TreeView<T> &tv, int2 root, const T (&q)[D], bool active, int k,
                                           int sorted, T seed2, T *drow, int64_t *irow, int64_t idx_base, Counters &c,
                                           T *buf_d, int32_t *buf_s, int tid, int nthreads, StackEntry<T, D> *stk,
-                                          int &found, int G = 1, int sub = 0, int gbase = 0) {
+                                          int &found, int G = 1, int sub = 0, int gbase = 0, const T *mass = nullptr,
+                                          T *rho_out = nullptr) {
     T bd[KMAX];
     int32_t bs[KMAX];
     const T init = seed2 < T(FLT_MAX) ? seed2 : T(FLT_MAX);
@@ -668,7 +699,10 @@ __device__ __forceinline__ bool knn_stack(const TreeView<T> &tv, int2 root, cons
         for (int j = 0; j < KMAX; j++) found += bs[j] >= 0 ? 1 : 0;
         return false;
     }
-    if (sub == 0) knn_write_row<T, KMAX>(tv, bd, bs, k, sorted, drow, irow, idx_base);
+    if (sub == 0) {
+        if constexpr (EPI == 0) knn_write_row<T, KMAX>(tv, bd, bs, k, sorted, drow, irow, idx_base);
+        else sph_row<T, KMAX>(tv, bd, bs, k, mass, idx_base, drow, rho_out);  // EPI 1: h to *drow
+    }
     return true;
 }
 

@@ -142,6 +142,26 @@ class KDTree(int):
                           _vp(stats), _stream()), "knn_search")
         return dist, idx
 
+    def sph_density(self, q, k: int = 32, mass=None):
+        """SPH smoothing length and density of device queries, from their k
+        nearest neighbours in one fused kernel (ckd_sph_density): returns
+        (h (nq,), rho (nq,)) on the GPU with h = r_k / 2 and rho the cubic
+        spline sum over the k neighbours.  mass: one per tree point in the
+        original order, or None for unit masses.  3-D trees only."""
+        self._check_alive()
+        q = self._queries(q)
+        nq = q.shape[0]
+        h = torch.empty(nq, dtype=self.torch_dtype, device=self.device)
+        rho = torch.empty(nq, dtype=self.torch_dtype, device=self.device)
+        if mass is not None:
+            mass = torch.as_tensor(mass, dtype=self.torch_dtype, device=self.device).contiguous()
+            if mass.numel() != self.n:
+                raise ValueError("mass needs one value per tree point")
+        if nq:
+            check(lib.ckd_sph_density(self._ptr, _vp(q), nq, int(k), _vp(mass), _vp(h), _vp(rho), _stream()),
+                  "sph_density")
+        return h, rho
+
     def ball(self, q, radius: float, sorted: bool = False):
         """Ball search of device queries: CSR (offsets (nq+1), dist, idx) on the GPU."""
         self._check_alive()

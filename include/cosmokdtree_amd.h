@@ -113,6 +113,27 @@ int ckd_get_info(const ckd_tree *tree, ckd_tree_info *info);
 int ckd_knn(const ckd_tree *tree, const void *queries, int64_t nq, int32_t k, int32_t sorted, void *dist,
             int64_t *idx, const float *seed_r, int64_t *stats, void *stream);
 
+/*
+ * SPH smoothing length and density of nq query rows [dev, (nq, 3), prec],
+ * fused into the kNN traversal: no kNN rows are written.  No counterpart in
+ * the reference library; it is the step its callers take next with the
+ * k = 32 neighbours (the cosmology use named in README.md:18), in one
+ * kernel instead of knn_search (cosmokdtree.f90:504-540) followed by a pass
+ * over the (nq, k) rows.  For each query, with r_1 <= ... <= r_k the
+ * distances ckd_knn returns:
+ *   h   = r_k / 2  (the cubic spline's support 2h reaches the k-th point)
+ *   rho = sum_j m_j W(r_j, h),  W(r, h) = w(r / h) / (pi h^3),
+ *   w(u) = 1 - 1.5 u^2 + 0.75 u^3 (u < 1), 0.25 (2 - u)^3 (u < 2), else 0
+ * (Monaghan & Lattanzio 1985); a query that is a tree point counts itself.
+ * mass [dev, n, prec] indexed by original point index, or NULL for unit
+ * masses.  Outputs h, rho [dev, nq, prec].  Needs a 3-D tree of depth <= 64
+ * and 2 <= k <= min(64, n); k coincident points give h = 0 and a
+ * non-finite rho.  h equals half of ckd_knn's k-th distance bit for bit;
+ * rho is summed in order of increasing distance.
+ */
+int ckd_sph_density(const ckd_tree *tree, const void *queries, int64_t nq, int32_t k, const void *mass, void *h,
+                    void *rho, void *stream);
+
 #define CKD_KNN_AUTO (-1)
 #define CKD_KNN_LANE 0   /* one traversal per query */
 #define CKD_KNN_WAVE 1   /* the queries of a wave share one traversal (parent pointers, any depth) */
