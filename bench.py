@@ -197,7 +197,8 @@ def main():
         del d, i
     del d_plain
     kernel_ms = plain_kernel_ms
-    if learned_info and learned_info["distances_identical"]:
+    use_learned = bool(learned_info and learned_info["distances_identical"] and learned_info["seeded_qps"] > qps)
+    if use_learned:
         qps, ms_step = learned_info["seeded_qps"], learned_info["seeded_ms_per_step"]
         kernel_ms = learned_info["traversal_kernel_ms"]
     # roofline of the dominant kernel: algorithmic bytes of one launch (all N
@@ -310,10 +311,11 @@ def main():
         "dtype": "f32",
         "data": "synthetic: clustered periodic box (seeded numpy PCG64), no dataset",
         "config": {"workload": "self-query kNN k=32 sorted, clustered periodic box N=2^24 L=100 halo 50%, "
-                               "float32, leaf 16; value = learned-seed path (features + MLP + seeded "
-                               "traversal) when its distances equal the plain path's, else the plain path",
+                               "float32, leaf 16; value = the faster of the plain path and the learned-seed "
+                               "path (features + MLP + seeded traversal, counted only when its distances "
+                               "equal the plain path's)",
                    "n_points": N, "queries": N, "k": K, "leaf": 16, "boxsize": L,
-                   "path": "learned-seed" if learned_info and learned_info["distances_identical"] else "plain"},
+                   "path": "learned-seed" if use_learned else "plain"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": kernel_name, "kernel_ms": kernel_ms, "bytes_per_query": bq},

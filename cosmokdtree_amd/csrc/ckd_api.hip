@@ -332,9 +332,8 @@ int ckd_knn_config(ckd_tree *t, int32_t kernel, int32_t stack_depth, int32_t que
         set_error("stack depth must be 0, 32 or 64, got %d", stack_depth);
         return CKD_ERR_ARG;
     }
-    if (queries_per_wave != 0 && queries_per_wave != 1 && queries_per_wave != 4 && queries_per_wave != 16 &&
-        queries_per_wave != 64) {
-        set_error("queries per wave must be 0, 1, 4, 16 or 64, got %d", queries_per_wave);
+    if (queries_per_wave < 0 || queries_per_wave > 64 || (queries_per_wave & (queries_per_wave - 1)) != 0) {
+        set_error("queries per wave must be 0 or a power of two up to 64, got %d", queries_per_wave);
         return CKD_ERR_ARG;
     }
     t->knn_kernel = kernel;

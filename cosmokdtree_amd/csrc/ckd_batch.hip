@@ -109,9 +109,9 @@ __global__ void k_bfill(int64_t *p, int64_t n, int64_t v) {
 // held by G lanes.
 template <typename T, int D, bool PER, int KMAX, int SD, int NT>
 __device__ __forceinline__ void batch_knn_stack(const TreeView<T> &tv, const BatchItem &w, const T *queries,
-                                                const int32_t *order, int k, int64_t base, int64_t kbase,
-                                                int64_t q0set, int sorted, T *kdist, int64_t *kidx, int G, T *buf_d,
-                                                int32_t *buf_s, StackEntry<T, D> *stk) {
+                                                const int32_t *order, const int32_t *leaf_of, int k, int64_t base,
+                                                int64_t kbase, int64_t q0set, int sorted, T *kdist, int64_t *kidx,
+                                                int G, T *buf_d, int32_t *buf_s, StackEntry<T, D> *stk) {
     constexpr int CB = KMAX < 16 ? 0 : (sizeof(T) == 4 ? 16 : 8);
     const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE, sub = lane & (G - 1),
               gbase = lane & ~(G - 1);
@@ -128,8 +128,9 @@ __device__ __forceinline__ void batch_knn_stack(const TreeView<T> &tv, const Bat
         const int64_t row = kbase + (qi - q0set) * k;
         Counters c;
         int found = 0;
+        const int32_t leaf = G == 1 && active && root.y < 0 ? leaf_of[qi] : -1;  // list prefill
         knn_stack<T, D, PER, KMAX, SD, CB>(tv, root, q, active, k, sorted, T(FLT_MAX), kdist + row, kidx + row, base,
-                                          c, buf_d, buf_s, threadIdx.x, NT, stk, found, G, sub, gbase);
+                                          c, buf_d, buf_s, threadIdx.x, NT, stk, found, G, sub, gbase, leaf);
     }
 }
 
@@ -142,7 +143,7 @@ __global__ __launch_bounds__(NT, 3) void k_batch(TreeView<T> tv, const int64_t *
                                               const int64_t *knn_off, int sorted, T *kdist, int64_t *kidx,
                                               int64_t *ball_counts, const int64_t *ball_off, T *bdist,
                                               int64_t *bidx, int64_t capacity, int64_t *overflow,
-                                              const int32_t *order, int Grt) {
+                                              const int32_t *order, const int32_t *leaf_of, int Grt) {
     const int G = GC ? GC : Grt;
     __shared__ int64_t s_item;
     constexpr int CBMAX = sizeof(T) == 4 ? 16 : 8;
@@ -161,16 +162,16 @@ __global__ __launch_bounds__(NT, 3) void k_batch(TreeView<T> tv, const int64_t *
             StackEntry<T, D> *ws = stk[threadIdx.x / WAVE];
             const int64_t base = set_off[w.set], kb = knn_off[w.set], q0s = qoff[w.set];
             if (k <= 8)
-                batch_knn_stack<T, D, PER, 8, SD, NT>(tv, w, queries, order, k, base, kb, q0s, sorted, kdist, kidx, G,
+                batch_knn_stack<T, D, PER, 8, SD, NT>(tv, w, queries, order, leaf_of, k, base, kb, q0s, sorted, kdist, kidx, G,
                                                       buf_d, buf_s, ws);
             else if (k <= 16)
-                batch_knn_stack<T, D, PER, 16, SD, NT>(tv, w, queries, order, k, base, kb, q0s, sorted, kdist, kidx, G,
+                batch_knn_stack<T, D, PER, 16, SD, NT>(tv, w, queries, order, leaf_of, k, base, kb, q0s, sorted, kdist, kidx, G,
                                                        buf_d, buf_s, ws);
             else if (k <= 32)
-                batch_knn_stack<T, D, PER, 32, SD, NT>(tv, w, queries, order, k, base, kb, q0s, sorted, kdist, kidx, G,
+                batch_knn_stack<T, D, PER, 32, SD, NT>(tv, w, queries, order, leaf_of, k, base, kb, q0s, sorted, kdist, kidx, G,
                                                        buf_d, buf_s, ws);
             else
-                batch_knn_stack<T, D, PER, 64, SD, NT>(tv, w, queries, order, k, base, kb, q0s, sorted, kdist, kidx, G,
+                batch_knn_stack<T, D, PER, 64, SD, NT>(tv, w, queries, order, leaf_of, k, base, kb, q0s, sorted, kdist, kidx, G,
                                                        buf_d, buf_s, ws);
             continue;
         }
@@ -231,12 +232,12 @@ int run_t(ckd_batch *b, const T *queries, T *kdist, int64_t *kidx, int64_t *ball
             k_batch<T, D, PER, NT, SD, 1><<<b->grid, NT, 0, s>>>(tv, t->set_off, queries, b->items_a, b->n_a,
                                                                  b->counters, b->kset, b->rset, b->qoff, b->knn_off,
                                                                  b->sorted, kdist, kidx, b->counts, ball_off, bdist,
-                                                                 bidx, b->capacity, b->counters + 2, order, 1);
+                                                                 bidx, b->capacity, b->counters + 2, order, b->leaf_of, 1);
         else
             k_batch<T, D, PER, NT, SD, 0><<<b->grid, NT, 0, s>>>(tv, t->set_off, queries, b->items_a, b->n_a,
                                                                  b->counters, b->kset, b->rset, b->qoff, b->knn_off,
                                                                  b->sorted, kdist, kidx, b->counts, ball_off, bdist,
-                                                                 bidx, b->capacity, b->counters + 2, order, G);
+                                                                 bidx, b->capacity, b->counters + 2, order, b->leaf_of, G);
     }
     CKD_HIP(hipGetLastError());
     if (b->has_ball) {
@@ -247,7 +248,7 @@ int run_t(ckd_batch *b, const T *queries, T *kdist, int64_t *kidx, int64_t *ball
                                                                  b->counters + 1, b->kset, b->rset, b->qoff,
                                                                  b->knn_off, b->sorted, kdist, kidx, b->counts,
                                                                  ball_off, bdist, bidx, b->capacity, b->counters + 2,
-                                                                 nullptr, 1);
+                                                                 nullptr, nullptr, 1);
         CKD_HIP(hipGetLastError());
     }
     return CKD_OK;
@@ -408,9 +409,9 @@ int ckd_batch_plan(const ckd_tree *t, const int64_t *qoff, const int32_t *k, con
 
 int ckd_batch_config(ckd_batch *b, int32_t knn_kernel, int32_t queries_per_wave, int32_t stack_depth) {
     if (!b || (knn_kernel != CKD_KNN_LANE && knn_kernel != CKD_KNN_STACK) ||
-        (queries_per_wave != 1 && queries_per_wave != 4 && queries_per_wave != 16 && queries_per_wave != 64) ||
+        (queries_per_wave < 1 || queries_per_wave > 64 || (queries_per_wave & (queries_per_wave - 1)) != 0) ||
         (stack_depth != 0 && stack_depth != 32 && stack_depth != 64)) {
-        set_error("ckd_batch_config: kernel CKD_KNN_LANE or CKD_KNN_STACK, queries per wave 1, 4, 16 or 64, "
+        set_error("ckd_batch_config: kernel CKD_KNN_LANE or CKD_KNN_STACK, queries per wave a power of two up to 64, "
                   "stack depth 0, 32 or 64");
         return CKD_ERR_ARG;
     }

@@ -1,18 +1,27 @@
-// ckd_query_f32.hip -- the float instantiation of the query kernels.
+// ckd_query_f32.hip -- the float instantiation of the ball, box and SPH kernels, and the
+// kNN dispatch to ckd_knn_f32_per.hip / ckd_knn_f32_open.hip.
 #include "ckd_query_impl.h"
 
 namespace ckd {
 
+int query_knn_f32_per(const ckd_tree *, const void *, int64_t, int, int, const float *, const float *, float,
+                      int, const uint16_t *, void *, int64_t *, int64_t *, hipStream_t);
+int query_knn_f32_open(const ckd_tree *, const void *, int64_t, int, int, const float *, const float *, float,
+                       int, const uint16_t *, void *, int64_t *, int64_t *, hipStream_t);
+
 int query_knn_f32(const ckd_tree *t, const void *q, int64_t nq, int k, int sorted, const float *seed, void *dist,
                   int64_t *idx, int64_t *stats, hipStream_t s) {
-    return knn_t<float>(t, q, nq, k, sorted, seed, nullptr, dist, idx, stats, s);
+    return t->periodic
+               ? query_knn_f32_per(t, q, nq, k, sorted, seed, nullptr, 0.0f, 0, nullptr, dist, idx, stats, s)
+               : query_knn_f32_open(t, q, nq, k, sorted, seed, nullptr, 0.0f, 0, nullptr, dist, idx, stats, s);
 }
 
 int query_knn_learned_f32(const ckd_tree *t, const void *q, int64_t nq, int k, int sorted, const float *params,
                           float margin, int G, const uint16_t *table, void *dist, int64_t *idx, int64_t *stats,
                           hipStream_t s) {
-    const Learned ln{params, margin, G, table};
-    return knn_t<float>(t, q, nq, k, sorted, nullptr, &ln, dist, idx, stats, s);
+    return t->periodic
+               ? query_knn_f32_per(t, q, nq, k, sorted, nullptr, params, margin, G, table, dist, idx, stats, s)
+               : query_knn_f32_open(t, q, nq, k, sorted, nullptr, params, margin, G, table, dist, idx, stats, s);
 }
 
 int query_ball_f32(const ckd_tree *t, const void *q, int64_t nq, double r, int64_t *counts, const int64_t *off,

@@ -180,8 +180,8 @@ __global__ __launch_bounds__(QT, 3) void k_knn_w(TreeView<T> tv, const T *querie
 template <typename T, int D, bool PER, int KMAX, int SD, int GC>
 __global__ __launch_bounds__(QT, 3) void k_knn_s(TreeView<T> tv, int2 root, const T *queries, int64_t nq,
                                               const int64_t *nq_dev, int k, int sorted, const float *seed,
-                                              const int32_t *order, T *dout, int64_t *iout, int64_t *stats,
-                                              Retry rt, int Grt) {
+                                              const int32_t *order, const int32_t *leaf_of, T *dout,
+                                              int64_t *iout, int64_t *stats, Retry rt, int Grt) {
     const int G = GC ? GC : Grt;
     constexpr int CB = KMAX < 16 ? 0 : (sizeof(T) == 4 ? 16 : 8);
     __shared__ T buf_d[CB > 0 ? CB * QT : 1];
@@ -205,9 +205,11 @@ __global__ __launch_bounds__(QT, 3) void k_knn_s(TreeView<T> tv, int2 root, cons
         seed2 = (T)sr * (T)sr;
     }
     int found = 0;
+    int32_t leaf = -1;  // prefill from the leaf's ancestor (one lane per query, inner root)
+    if (G == 1 && active && root.y < 0) leaf = leaf_of ? leaf_of[qi] : find_leaf<T, D>(tv, q, 0);
     const bool done = knn_stack<T, D, PER, KMAX, SD, CB>(tv, root, q, active, k, sorted, seed2, dout + qi * k,
                                                           iout + qi * k, 0, c, buf_d, buf_s, threadIdx.x, QT,
-                                                          stk[threadIdx.x / WAVE], found, G, sub, gbase);
+                                                          stk[threadIdx.x / WAVE], found, G, sub, gbase, leaf);
     if (!done && sub == 0) send_retry(rt, t, qi, sr, found, k);
     if (stats) {
         if (!active || sub != 0) c = Counters();
@@ -219,7 +221,8 @@ __global__ __launch_bounds__(QT, 3) void k_knn_s(TreeView<T> tv, int2 root, cons
 // walk (sph_row): writes h[qi], rho[qi] instead of the k-row.
 template <typename T, bool PER, int KMAX, int SD>
 __global__ __launch_bounds__(QT, 3) void k_sph_s(TreeView<T> tv, int2 root, const T *queries, int64_t nq, int k,
-                                              const int32_t *order, const T *mass, T *hout, T *rhoout) {
+                                              const int32_t *order, const int32_t *leaf_of, const T *mass,
+                                              T *hout, T *rhoout) {
     constexpr int D = 3;
     constexpr int CB = KMAX < 16 ? 0 : (sizeof(T) == 4 ? 16 : 8);
     __shared__ T buf_d[CB > 0 ? CB * QT : 1];
@@ -235,8 +238,10 @@ __global__ __launch_bounds__(QT, 3) void k_sph_s(TreeView<T> tv, int2 root, cons
 #pragma unroll
     for (int d = 0; d < D; d++) q[d] = active ? queries[qi * D + d] : T(0);
     int found = 0;
+    int32_t leaf = -1;
+    if (active && root.y < 0) leaf = leaf_of ? leaf_of[qi] : find_leaf<T, D>(tv, q, 0);
     knn_stack<T, D, PER, KMAX, SD, CB, 1>(tv, root, q, active, k, 1, T(FLT_MAX), hout + qi, nullptr, 0, c, buf_d,
-                                         buf_s, threadIdx.x, QT, stk[threadIdx.x / WAVE], found, 1, 0, 0, mass,
+                                         buf_s, threadIdx.x, QT, stk[threadIdx.x / WAVE], found, 1, 0, 0, leaf, mass,
                                          rhoout + qi);
 }
 
@@ -415,15 +420,15 @@ struct Learned {
 template <typename T, int D, bool PER, int KMAX>
 int launch_wave(const ckd_tree *t, const TreeView<T> &tv, int kind, int sd, int qpw, unsigned g, const T *q,
                 int64_t nq, const int64_t *nq_dev, int k, int sorted, const float *seed, const int32_t *order,
-                T *dist, int64_t *idx, int64_t *stats, const Retry &rt, hipStream_t s) {
+                const int32_t *leaf_of, T *dist, int64_t *idx, int64_t *stats, const Retry &rt, hipStream_t s) {
     if (kind == CKD_KNN_STACK) {
         if constexpr (D <= 3) {
             const int2 root = make_int2(0, t->nnodes > 1 ? -1 : (int32_t)t->n);
             const int G = WAVE / qpw;
             const unsigned gs = (unsigned)ceil_div(nq * G, QT);
 #define CKD_KNN_S(SD_, GC_)                                                                                  \
-    k_knn_s<T, D, PER, KMAX, SD_, GC_><<<gs, QT, 0, s>>>(tv, root, q, nq, nq_dev, k, sorted, seed, order, dist, idx, \
-                                                         stats, rt, G)
+    k_knn_s<T, D, PER, KMAX, SD_, GC_><<<gs, QT, 0, s>>>(tv, root, q, nq, nq_dev, k, sorted, seed, order, leaf_of, \
+                                                         dist, idx, stats, rt, G)
             if (sd <= 32) {
                 if (G == 1) CKD_KNN_S(32, 1);
                 else CKD_KNN_S(32, 0);
@@ -474,7 +479,7 @@ int knn_launch_k(const ckd_tree *t, const TreeView<T> &tv, int kind, int sd, int
     }
     if (!seed) {
         if ((rc = tm.start(s))) return rc;
-        if ((rc = launch_wave<T, D, PER, KMAX>(t, tv, kind, sd, qpw, g, q, nq, nullptr, k, sorted, nullptr, order, dist, idx,
+        if ((rc = launch_wave<T, D, PER, KMAX>(t, tv, kind, sd, qpw, g, q, nq, nullptr, k, sorted, nullptr, order, leaf_of, dist, idx,
                                                stats, Retry(), s)))
             return rc;
         return tm.stop(s);
@@ -505,7 +510,7 @@ int knn_launch_k(const ckd_tree *t, const TreeView<T> &tv, int kind, int sd, int
     r1.flag = flag;
     r1.seed_next = seed_next;
     if ((rc = tm.start(s))) return rc;
-    if ((rc = launch_wave<T, D, PER, KMAX>(t, tv, kind, sd, qpw, g, q, nq, nullptr, k, sorted, seed, order, dist, idx,
+    if ((rc = launch_wave<T, D, PER, KMAX>(t, tv, kind, sd, qpw, g, q, nq, nullptr, k, sorted, seed, order, leaf_of, dist, idx,
                                            stats, r1, s)))
         return rc;
     if ((rc = tm.stop(s))) return rc;
@@ -514,7 +519,7 @@ int knn_launch_k(const ckd_tree *t, const TreeView<T> &tv, int kind, int sd, int
     Retry r2;
     r2.list = list2;
     r2.count = count2;
-    if ((rc = launch_wave<T, D, PER, KMAX>(t, tv, kind, sd, qpw, g, q, nq, off + nq, k, sorted, seed_next, list, dist, idx,
+    if ((rc = launch_wave<T, D, PER, KMAX>(t, tv, kind, sd, qpw, g, q, nq, off + nq, k, sorted, seed_next, list, leaf_of, dist, idx,
                                            stats, r2, s)))
         return rc;
     k_knn<T, D, PER, KMAX><<<g, QT, 0, s>>>(tv, q, nq, count2, k, sorted, nullptr, list2, nullptr, dist, idx, stats);
@@ -645,9 +650,9 @@ int sph_launch(const ckd_tree *t, const T *q, int64_t nq, int k, const T *mass, 
         set_error("the SPH step needs a 3-D tree of depth <= 64 and fewer than 2^31 points");
         return CKD_ERR_LIMIT;
     }
-    const int32_t *order = nullptr;
+    const int32_t *order = nullptr, *leaf_of = nullptr;
     if (nq >= ORDER_MIN_QUERIES && nq < INT32_MAX && t->nnodes > 1) {
-        const int rc = order_queries<T, 3>(t, tv, q, nq, &order, nullptr, s);
+        const int rc = order_queries<T, 3>(t, tv, q, nq, &order, &leaf_of, s);
         if (rc) return rc;
     }
     const int2 root = make_int2(0, t->nnodes > 1 ? -1 : (int32_t)t->n);
@@ -657,8 +662,8 @@ int sph_launch(const ckd_tree *t, const T *q, int64_t nq, int k, const T *mass, 
     if (rc) return rc;
 #define CKD_SPH(KM)                                                                                        \
     do {                                                                                                  \
-        if (t->depth <= 32) k_sph_s<T, PER, KM, 32><<<g, QT, 0, s>>>(tv, root, q, nq, k, order, mass, h, rho); \
-        else k_sph_s<T, PER, KM, 64><<<g, QT, 0, s>>>(tv, root, q, nq, k, order, mass, h, rho);              \
+        if (t->depth <= 32) k_sph_s<T, PER, KM, 32><<<g, QT, 0, s>>>(tv, root, q, nq, k, order, leaf_of, mass, h, rho); \
+        else k_sph_s<T, PER, KM, 64><<<g, QT, 0, s>>>(tv, root, q, nq, k, order, leaf_of, mass, h, rho);              \
     } while (0)
     if (k <= 8) CKD_SPH(8);
     else if (k <= 16) CKD_SPH(16);
@@ -749,14 +754,15 @@ int box_launch(const ckd_tree *t, const T *b, int64_t nb, int64_t *counts, const
         case 8: { constexpr int D_ = 8; return CALL; } \
     }
 
-template <typename T>
-int knn_t(const ckd_tree *t, const void *q, int64_t nq, int k, int sorted, const float *seed, const Learned *ln,
-          void *dist, int64_t *idx, int64_t *stats, hipStream_t s) {
-    if (t->periodic) {
-        CKD_DIM_SWITCH(t->ndim, (knn_launch<T, D_, true>(t, (const T *)q, nq, k, sorted, seed, ln, (T *)dist, idx, stats, s)))
-    } else {
-        CKD_DIM_SWITCH(t->ndim, (knn_launch<T, D_, false>(t, (const T *)q, nq, k, sorted, seed, ln, (T *)dist, idx, stats, s)))
-    }
+// kNN of one precision and periodicity (each pair is its own translation
+// unit, ckd_knn_<prec>_<per|open>.hip, so they compile in parallel);
+// params != nullptr asks for learned seeds.
+template <typename T, bool PER>
+int knn_per(const ckd_tree *t, const void *q, int64_t nq, int k, int sorted, const float *seed, const float *params,
+            float margin, int G, const uint16_t *table, void *dist, int64_t *idx, int64_t *stats, hipStream_t s) {
+    const Learned ln{params, margin, G, table};
+    const Learned *lp = params ? &ln : nullptr;
+    CKD_DIM_SWITCH(t->ndim, (knn_launch<T, D_, PER>(t, (const T *)q, nq, k, sorted, seed, lp, (T *)dist, idx, stats, s)))
     return CKD_ERR_ARG;
 }
 

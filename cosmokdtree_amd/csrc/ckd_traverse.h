@@ -410,12 +410,17 @@ __device__ __forceinline__ void load_rows(const T *pts, int32_t p, T (&x)[PB][D]
 
 // Leaf scan shared by the stack kernels: rows [pb, pe), candidates buffered
 // per lane in LDS (CB > 0 slots per lane, slot j of lane tid at
-// j * nthreads + tid) or inserted directly (CB == 0).
+// j * nthreads + tid) or inserted directly (CB == 0).  A lane skips the leaf
+// when its rows lie in the lane's prefilled range [xs, xs + xm) (knn_stack):
+// a leaf's rows are either all inside an ancestor's row range or all
+// outside it.
 template <typename T, int D, bool PER, int KMAX, int CB>
 __device__ __forceinline__ void wave_scan_rows(const TreeView<T> &tv, int32_t pb, int32_t pe, const T (&q)[D],
-                                               bool run, T (&bd)[KMAX], int32_t (&bs)[KMAX], Counters &c,
-                                               T *buf_d, int32_t *buf_s, int tid, int nthreads) {
+                                               bool run_q, T (&bd)[KMAX], int32_t (&bs)[KMAX], Counters &c,
+                                               T *buf_d, int32_t *buf_s, int tid, int nthreads, int32_t xs,
+                                               uint32_t xm) {
     constexpr int PB = sizeof(T) == 4 ? 8 : 4;  // rows loaded per batch (CB is a multiple)
+    const bool run = run_q && (uint32_t)(pb - xs) >= xm;
     if constexpr (CB > 0) {
         // leaves longer than CB rows go in chunks of CB, so a lane never
         // buffers more than CB candidates; candidates are inserted in row
@@ -568,13 +573,14 @@ template <typename T, int D, bool PER, int KMAX, int SD, int CB, bool CHECK>
 __device__ __forceinline__ bool stack_walk(const TreeView<T> &tv, const T (&q)[D], bool run, T safe2, T (&bd)[KMAX],
                                            int32_t (&bs)[KMAX], int32_t &a, int32_t &b, int &sp, Counters &c,
                                            T *buf_d, int32_t *buf_s, int tid, int nthreads, StackEntry<T, D> *stk,
-                                           int G, int sub, int gbase) {
+                                           int G, int sub, int gbase, int32_t xs, uint32_t xm) {
     for (;;) {
         if (CHECK && !__any(run && !(safe2 >= bd[KMAX - 1]))) return false;
         c.visited++;
         bool descend = false;
         if (b >= 0) {
-            if (G == 1) wave_scan_rows<T, D, PER, KMAX, CB>(tv, a, a + b, q, run, bd, bs, c, buf_d, buf_s, tid, nthreads);
+            if (G == 1)
+                wave_scan_rows<T, D, PER, KMAX, CB>(tv, a, a + b, q, run, bd, bs, c, buf_d, buf_s, tid, nthreads, xs, xm);
             else group_scan_rows<T, D, PER, KMAX>(tv, a, a + b, q, run, bd, bs, c, G, sub, gbase);
         } else {
             T b0[2 * D], b1[2 * D];
@@ -664,12 +670,20 @@ __device__ __forceinline__ void sph_row(const TreeView<T> &tv, const T (&bd)[KMA
 // rows, keep identical lists, and lane sub == 0 writes the row.  EPI = 1
 // writes the SPH smoothing length to *drow and density to *rho_out instead
 // of the row (sph_row).
+//
+// With G == 1 and the query's leaf given (leaf >= 0), the lane first fills
+// its list from the rows of the leaf's smallest ancestor holding at least k
+// rows (or the root): every lane of the wave inserts about as many rows, so
+// the rounds are balanced, and the list's k-th distance is already close
+// when the shared walk starts, which then skips those rows.  Far points
+// that the walk's first leaves would have inserted, only to be displaced,
+// are never inserted.
 template <typename T, int D, bool PER, int KMAX, int SD, int CB, int EPI = 0>
 __device__ __forceinline__ bool knn_stack(const TreeView<T> &tv, int2 root, const T (&q)[D], bool active, int k,
                                           int sorted, T seed2, T *drow, int64_t *irow, int64_t idx_base, Counters &c,
                                           T *buf_d, int32_t *buf_s, int tid, int nthreads, StackEntry<T, D> *stk,
-                                          int &found, int G = 1, int sub = 0, int gbase = 0, const T *mass = nullptr,
-                                          T *rho_out = nullptr) {
+                                          int &found, int G = 1, int sub = 0, int gbase = 0, int32_t leaf = -1,
+                                          const T *mass = nullptr, T *rho_out = nullptr) {
     T bd[KMAX];
     int32_t bs[KMAX];
     const T init = seed2 < T(FLT_MAX) ? seed2 : T(FLT_MAX);
@@ -681,15 +695,57 @@ __device__ __forceinline__ bool knn_stack(const TreeView<T> &tv, int2 root, cons
     }
     int sp = 0;
     int32_t a = root.x, b = root.y;  // wave-uniform: the subtree to enter
+    int32_t xs = 0;                  // prefilled rows [xs, xs + xm) of this lane
+    uint32_t xm = 0;
+    if (G == 1) {
+        if (run && leaf >= 0) {
+            int32_t node = leaf;
+            Range rg = tv.range[node];
+            while (rg.count < k) {
+                const int32_t par = tv.nodes[node].parent;
+                if (par < 0) break;
+                node = par;
+                rg = tv.range[node];
+            }
+            xs = (int32_t)rg.start;
+            xm = (uint32_t)rg.count;
+        }
+        // rows xs + r .. xs + r + PB - 1 are read as one block; past the
+        // range they are masked, and stay within the PTS_PAD spare rows
+        // (xs + xm <= n, r < max xm <= n).  Two rows per block: more would
+        // hold row registers beside the list past the 128 VGPRs of four
+        // waves per SIMD.
+        constexpr int PB = 2;
+        const int mmax = __reduce_max_sync(~0ull, (int)xm);
+#pragma unroll 1
+        for (int r = 0; r < mmax; r += PB) {
+            T x[PB][D];
+            const T *src = tv.pts + (int64_t)(xs + r) * pstride(D);
+#pragma unroll
+            for (int j = 0; j < PB; j++)
+#pragma unroll
+                for (int d = 0; d < D; d++) x[j][d] = src[j * pstride(D) + d];
+#pragma unroll
+            for (int j = 0; j < PB; j++) {
+                // unconditional: list_insert leaves the list as it is for a
+                // distance at or above the k-th (a masked row inserts +inf),
+                // which keeps the list in place instead of a selected copy
+                const T d = r + j < (int)xm ? dist2<T, D, PER>(x[j], q, tv.L) : T(INFINITY);
+                c.inserted += d < bd[KMAX - 1] ? 1 : 0;
+                list_insert<T, KMAX>(bd, bs, d, xs + r + j);
+            }
+        }
+        c.tested += xm;
+    }
     if constexpr (PER) {
         const T safe2 = wrap_free_bound<T, D>(q, tv.L);
         if (!stack_walk<T, D, true, KMAX, SD, CB, true>(tv, q, run, safe2, bd, bs, a, b, sp, c, buf_d, buf_s, tid,
-                                                        nthreads, stk, G, sub, gbase))
+                                                        nthreads, stk, G, sub, gbase, xs, xm))
             stack_walk<T, D, false, KMAX, SD, CB, false>(tv, q, run, safe2, bd, bs, a, b, sp, c, buf_d, buf_s, tid,
-                                                         nthreads, stk, G, sub, gbase);
+                                                         nthreads, stk, G, sub, gbase, xs, xm);
     } else {
         stack_walk<T, D, false, KMAX, SD, CB, false>(tv, q, run, T(0), bd, bs, a, b, sp, c, buf_d, buf_s, tid,
-                                                     nthreads, stk, G, sub, gbase);
+                                                     nthreads, stk, G, sub, gbase, xs, xm);
     }
     if (!active) return true;
     if (bs[KMAX - 1] < 0 && init != T(FLT_MAX)) {

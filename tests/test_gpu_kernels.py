@@ -113,7 +113,8 @@ def test_knn_config_rejects(ckd):
             tree.knn_config("warp")
         with pytest.raises(Exception):
             tree.knn_config("stack", 48)
-        with pytest.raises(Exception):
-            tree.knn_config("stack", 0, 8)
+        for qpw in (3, 12, 128, -1):
+            with pytest.raises(Exception):
+                tree.knn_config("stack", 0, qpw)
     finally:
         tree.free()

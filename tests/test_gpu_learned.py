@@ -132,10 +132,12 @@ def test_seeded_knn_is_exact(L_, clustered):
 
 
 def test_trained_network_gains(L_, clustered, ckd):
-    """Train on an independent point set, then compare visits on this one.
-    The gain is measured with walks shared 64 queries per wave (what dense
-    query sets use): there the seed replaces the early loose bound of the
-    shared walk; a walk of its own already finds good candidates first."""
+    """Train on an independent point set, then search this one with the
+    learned seeds: the fallback rate is low and the results are the plain
+    ones, for the trained margin and for margins far too short or long.
+    (The walk's own ancestor prefill already gives each lane a tight early
+    bound, so the seed no longer removes many insertions; the counts are
+    printed, not asserted.)"""
     from cosmokdtree_amd.synthetic import clustered_periodic
     train_pts = clustered_periodic(1 << 18, 100.0, 0.5, seed=77, dtype=np.float32)
     ttree = ckd.kdtree._build(train_pts, 16, np.full(3, 100.0), keep_points=True)
@@ -155,8 +157,6 @@ def test_trained_network_gains(L_, clustered, ckd):
           f"per query; seeded: {s1[0] / s1[3]:.1f} nodes, {s1[1] / s1[3]:.1f} points, "
           f"{s1[4] / s1[3]:.1f} insertions; fallback {fallback:.4f}")
     assert fallback < 0.1
-    # the seed removes the insertions of candidates beyond it (re-runs included)
-    assert s1[4] < 0.85 * s0[4]
     # whatever the margin (nearly every seed short, or all far too long), the
     # results are the plain ones: short seeds go through both retry rounds
     table = L_.feature_table(L_.occupancy(tree))
