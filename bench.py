@@ -205,7 +205,8 @@ def main():
     # queries) over its average launch duration
     achieved = bq * N / (kernel_ms / 1e3) / 1e9
     sd = 32 if info.depth <= 32 else 64
-    kernel_name = f"k_knn_s<float,3,periodic,{K},{sd}>" if info.depth <= 64 else f"k_knn_w<float,3,periodic,{K}>"
+    kernel_name = (f"k_knn_s<float, 3, true, {K}, {sd}, 1>" if info.depth <= 64
+                   else f"k_knn_w<float, 3, true, {K}>")
 
     extra = dict(build_ms=build_ms, build_mpts_s=N / build_ms / 1e3, tree_nodes=int(info.nnodes),
                  tree_depth=int(info.depth), nodes_visited_per_query=float(st[0] / max(st[3], 1)),
@@ -295,7 +296,7 @@ def main():
         with open(tpath) as f:
             tr = json.load(f)
         if tr.get("n") == N and tr.get("k") == K:
-            traffic = tr.get("bytes_per_launch")
+            traffic = tr.get("seeded_bytes_per_launch" if use_learned else "plain_bytes_per_launch")
 
     out = {
         "metric": METRIC,
