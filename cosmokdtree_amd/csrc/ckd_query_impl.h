@@ -549,17 +549,19 @@ int knn_launch(const ckd_tree *t, const T *q, int64_t nq, int k, int sorted, con
         if (sd < t->depth) sd = 64;
         int qpw = t->knn_qpw > 0 ? t->knn_qpw : 64;
         if (kind < 0) {
-            // Automatic choice (measured on MI355X): queries at least a quarter
-            // as many as the points share walks 64 per wave; a sparse batch
-            // small enough to leave the GPU half idle runs one query per
-            // wave (its 64 lanes split the leaf rows); a large sparse batch
-            // runs one walk per query.
+            // Automatic choice (measured on MI355X, 2^18 and 2^22 clustered
+            // points, k = 8, 32, 64): a batch of at most 65536 queries with
+            // k <= 32 and fewer queries than a quarter of the points runs one
+            // query per wave (its 64 lanes split the leaf rows); otherwise
+            // queries at least 1/20 of the points, or k > 32, share walks 64
+            // per wave; a large sparse batch with k <= 32 runs one walk per
+            // query.
             if (!stack_ok<T, D>(t, 64)) kind = k > 8 ? CKD_KNN_WAVE : CKD_KNN_LANE;
-            else if (nq * 4 >= t->n) kind = CKD_KNN_STACK;
-            else if (nq <= 65536) {
+            else if (nq <= 65536 && k <= 32 && nq * 4 < t->n) {
                 kind = CKD_KNN_STACK;
                 if (t->knn_qpw == 0) qpw = 1;
-            } else kind = CKD_KNN_LANE;
+            } else if (nq * 20 >= t->n || k > 32) kind = CKD_KNN_STACK;
+            else kind = CKD_KNN_LANE;
         }
         if (kind == CKD_KNN_STACK && !stack_ok<T, D>(t, sd)) kind = CKD_KNN_WAVE;  // same results, any depth
         if (D > 3) kind = CKD_KNN_LANE;  // the wave kernels are built for ndim <= 3

@@ -710,17 +710,19 @@ __device__ __forceinline__ bool knn_stack(const TreeView<T> &tv, int2 root, cons
             xs = (int32_t)rg.start;
             xm = (uint32_t)rg.count;
         }
-        // rows xs + r .. xs + r + PB - 1 are read as one block; past the
-        // range they are masked, and stay within the PTS_PAD spare rows
-        // (xs + xm <= n, r < max xm <= n).  Two rows per block: more would
-        // hold row registers beside the list past the 128 VGPRs of four
-        // waves per SIMD.
+        // Rows xs + r .. xs + r + PB - 1 are read as one block, masked past
+        // the lane's range.  The wave runs to the largest range, so a lane
+        // past its own (r >= xm) reads its block from row 0 instead; within
+        // it, the block ends at row xs + xm - 1 + PB - 1 < n + PTS_PAD.  Two
+        // rows per block: more would hold row registers beside the list
+        // past the 128 VGPRs of four waves per SIMD.
         constexpr int PB = 2;
+        static_assert(PB <= PTS_PAD, "prefill block longer than the padding");
         const int mmax = __reduce_max_sync(~0ull, (int)xm);
 #pragma unroll 1
         for (int r = 0; r < mmax; r += PB) {
             T x[PB][D];
-            const T *src = tv.pts + (int64_t)(xs + r) * pstride(D);
+            const T *src = tv.pts + (int64_t)(r < (int)xm ? xs + r : 0) * pstride(D);
 #pragma unroll
             for (int j = 0; j < PB; j++)
 #pragma unroll

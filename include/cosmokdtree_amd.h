@@ -145,12 +145,14 @@ int ckd_sph_density(const ckd_tree *tree, const void *queries, int64_t nq, int32
  * stack_depth is 0 (automatic), 32 or 64, the LDS stack entries per wave of
  * CKD_KNN_STACK, which is used only when ndim <= 3 and the tree depth fits
  * the stack (otherwise CKD_KNN_WAVE runs); queries_per_wave is 0 (= 64), 1,
- * 4, 16 or 64: how many queries share one wave of CKD_KNN_STACK (each then
- * held by 64 / queries_per_wave lanes that split the leaf rows).  With
- * CKD_KNN_AUTO, a call with at least n/4 queries shares walks 64 per wave,
- * a sparser one of at most 65536 queries runs one query per wave, a larger
- * sparse one one walk per query.  Every choice returns identical results;
- * only the speed differs.  Used by the autotuner.
+ * 4, 16 or 64 (any power of two up to 64): how many queries share one wave
+ * of CKD_KNN_STACK (each then held by 64 / queries_per_wave lanes that split
+ * the leaf rows).  With CKD_KNN_AUTO, a call of at most 65536 queries with
+ * k <= 32 and fewer than n/4 queries runs one query per wave; otherwise a
+ * call with at least n/20 queries, or k > 32, shares walks 64 per wave; a
+ * larger sparse call with k <= 32 runs one walk per query.  Every choice
+ * returns identical results; only the speed differs.  Used by the
+ * autotuner.
  */
 int ckd_knn_config(ckd_tree *tree, int32_t kernel, int32_t stack_depth, int32_t queries_per_wave);
 
