@@ -3,16 +3,17 @@
 // Restates build_kdtree / build_kdtree_recursive (cosmokdtree.f90:124-451)
 // one tree level at a time.  Every node of a level that holds more than
 // `leafsize` points ("active") is split in the same pass:
-//   1. its point range is cut into chunks of CH points; one workgroup per
-//      chunk reduces per-axis min/max (with first positions) -- the
+//   1. its point range is cut into chunks of CH points; one wave per chunk
+//      reduces per-axis min/max (with first positions) -- the
 //      minval/maxval/minloc/maxloc of cosmokdtree.f90:349-350,372-373;
-//   2. one workgroup per node combines its chunks and applies the split
-//      rule: longest cell side, exact ties to the larger spread (:355-365),
+//   2. one wave per node combines its chunks and applies the split rule:
+//      longest cell side, exact ties to the larger spread (:355-365),
 //      midpoint 0.5*(lo+hi) (:367), slide to the min (:376-388) or max
 //      (:391-403) when every point lies on one side;
-//   3. a stable prefix-sum partition (points with x < mid first, :458-479;
-//      for a slide only the extreme point moves) scatters coordinates and
-//      indices into the other ping-pong buffer;
+//   3. a stable partition (points with x < mid first, :458-479; for a slide
+//      only the extreme point moves), one wave per chunk taking 64
+//      consecutive points per round and placing them by ballot ranks,
+//      scatters coordinates and indices into the other ping-pong buffer;
 //   4. both children are created as an adjacent pair with their cells
 //      (:418-422); children with <= leafsize points become leaves and their
 //      points are copied to the final AoS leaf buckets.
@@ -30,7 +31,7 @@ namespace {
 
 constexpr int CH = 2048;          // points per chunk
 constexpr int BT = 256;           // threads per workgroup
-constexpr int IPT = CH / BT;      // points per thread in a chunk
+constexpr int CPB = BT / WAVE;    // chunks (or nodes) per workgroup, one per wave
 
 struct Chunk {
     int64_t begin, end;  // positions in the working buffers
@@ -94,6 +95,15 @@ __device__ __forceinline__ void block_reduce_ext(Ext<T> (&e)[D], Ext<T> (*lds)[D
             for (int d = 0; d < D; d++) ext_merge(e[d], lds[i][d]);
 }
 
+// Reduce one Ext per axis across the wave; result valid in every lane.
+template <typename T, int D>
+__device__ __forceinline__ void wave_reduce_ext(Ext<T> (&e)[D]) {
+#pragma unroll
+    for (int d = 0; d < D; d++)
+#pragma unroll
+        for (int m = WAVE / 2; m >= 1; m >>= 1) ext_merge(e[d], ext_shfl_xor(e[d], m));
+}
+
 // Decision for one active node.
 template <typename T>
 struct Decision {
@@ -140,16 +150,20 @@ __global__ void k_chunk_fill(const int32_t *act, const Range *range, const int64
     }
 }
 
+// One wave per chunk: lanes take points p = begin + lane, + 64, ...
+// (coalesced column reads); strict comparisons keep each lane's first
+// position, ext_merge the smallest position on ties.
 template <typename T, int D>
 __global__ __launch_bounds__(BT) void k_chunk_stats(const Chunk *chunks, const int64_t *nchunks, const T *P, int64_t n,
                                                     Ext<T> *cstat) {
-    __shared__ Ext<T> lds[BT / WAVE][D];
-    if ((int64_t)blockIdx.x >= *nchunks) return;
-    const Chunk ch = chunks[blockIdx.x];
+    const int64_t ci = (int64_t)blockIdx.x * CPB + threadIdx.x / WAVE;
+    if (ci >= *nchunks) return;  // whole wave
+    const int lane = threadIdx.x & (WAVE - 1);
+    const Chunk ch = chunks[ci];
     Ext<T> e[D];
 #pragma unroll
     for (int d = 0; d < D; d++) e[d] = ext_empty<T>();
-    for (int64_t p = ch.begin + threadIdx.x; p < ch.end; p += BT) {
+    for (int64_t p = ch.begin + lane; p < ch.end; p += WAVE) {
 #pragma unroll
         for (int d = 0; d < D; d++) {
             T x = P[(int64_t)d * n + p];
@@ -157,28 +171,31 @@ __global__ __launch_bounds__(BT) void k_chunk_stats(const Chunk *chunks, const i
             if (x > e[d].mx) { e[d].mx = x; e[d].mxpos = p; }
         }
     }
-    block_reduce_ext<T, D>(e, lds);
-    if (threadIdx.x == 0)
+    wave_reduce_ext<T, D>(e);
+    if (lane == 0)
 #pragma unroll
-        for (int d = 0; d < D; d++) cstat[(int64_t)blockIdx.x * D + d] = e[d];
+        for (int d = 0; d < D; d++) cstat[ci * D + d] = e[d];
 }
 
+// One wave per active node: its lanes merge the node's chunk statistics,
+// lane 0 applies the split rule.
 template <typename T, int D>
-__global__ __launch_bounds__(BT) void k_node_decide(const int32_t *act, const int64_t *chunk_base, const int64_t *nch,
-                                                    const Ext<T> *cstat, NodeRec<T> *nodes, T *bbox, T *dbox,
-                                                    Decision<T> *dec, int root_from_data) {
-    __shared__ Ext<T> lds[BT / WAVE][D];
-    const int64_t a = blockIdx.x;
+__global__ __launch_bounds__(BT) void k_node_decide(const int32_t *act, int64_t nact, const int64_t *chunk_base,
+                                                    const int64_t *nch, const Ext<T> *cstat, NodeRec<T> *nodes,
+                                                    T *bbox, T *dbox, Decision<T> *dec, int root_from_data) {
+    const int64_t a = (int64_t)blockIdx.x * CPB + threadIdx.x / WAVE;
+    if (a >= nact) return;  // whole wave
+    const int lane = threadIdx.x & (WAVE - 1);
     const int32_t id = act[a];
     Ext<T> e[D];
 #pragma unroll
     for (int d = 0; d < D; d++) e[d] = ext_empty<T>();
     const int64_t c0 = chunk_base[a], c1 = c0 + nch[a];
-    for (int64_t c = c0 + threadIdx.x; c < c1; c += BT)
+    for (int64_t c = c0 + lane; c < c1; c += WAVE)
 #pragma unroll
         for (int d = 0; d < D; d++) ext_merge(e[d], cstat[c * D + d]);
-    block_reduce_ext<T, D>(e, lds);
-    if (threadIdx.x != 0) return;
+    wave_reduce_ext<T, D>(e);
+    if (lane != 0) return;
     T *bb = bbox + (int64_t)id * 2 * D;
     T *db = dbox + (int64_t)id * 2 * D;
 #pragma unroll
@@ -228,68 +245,73 @@ __global__ __launch_bounds__(BT) void k_node_decide(const int32_t *act, const in
     nodes[id].axis = axis;
 }
 
+// Left count of each chunk (one wave per chunk, 64 points per ballot);
+// chunk slots past the level's count get 0 for the scan over nub slots.
 template <typename T>
 __global__ __launch_bounds__(BT) void k_chunk_countleft(const Chunk *chunks, const int64_t *nchunks, int64_t nub,
                                                         const T *P, int64_t n, const Decision<T> *dec,
                                                         int64_t *cleft) {
-    __shared__ int64_t lds[BT / WAVE + 1];
-    if ((int64_t)blockIdx.x >= nub) return;
-    if ((int64_t)blockIdx.x >= *nchunks) {
-        if (threadIdx.x == 0) cleft[blockIdx.x] = 0;
+    const int64_t ci = (int64_t)blockIdx.x * CPB + threadIdx.x / WAVE;
+    if (ci >= nub) return;  // whole wave
+    const int lane = threadIdx.x & (WAVE - 1);
+    if (ci >= *nchunks) {
+        if (lane == 0) cleft[ci] = 0;
         return;
     }
-    const Chunk ch = chunks[blockIdx.x];
+    const Chunk ch = chunks[ci];
     const Decision<T> dc = dec[ch.act];
     const T *col = P + (int64_t)dc.axis * n;
     int64_t c = 0;
-    for (int64_t p = ch.begin + threadIdx.x; p < ch.end; p += BT) c += goes_left(dc, col[p], p) ? 1 : 0;
-    int64_t tot;
-    block_exclusive_scan<int64_t, BT>(c, lds, &tot);
-    if (threadIdx.x == 0) cleft[blockIdx.x] = tot;
+    for (int64_t base = ch.begin; base < ch.end; base += WAVE) {
+        const int64_t p = base + lane;
+        const bool f = p < ch.end && goes_left(dc, col[p < ch.end ? p : base], p);
+        c += __popcll(__ballot(f));
+    }
+    if (lane == 0) cleft[ci] = c;
 }
 
+// Stable partition of one chunk by one wave: rounds of 64 consecutive
+// points; a point's place among the round's left (right) points is its
+// ballot rank, added to the left (right) points of earlier rounds, earlier
+// chunks of the node (cleft_off) and, for the right side, the node's left
+// count.  Reads and writes of a round are contiguous runs.
 template <typename T, int D, typename IdxT>
 __global__ __launch_bounds__(BT) void k_chunk_scatter(const Chunk *chunks, const int64_t *nchunks,
                                                       const int64_t *chunk_base, const int64_t *nch,
                                                       const int32_t *act, const Range *range, const T *P,
                                                       const IdxT *I, T *P2, IdxT *I2, int64_t n,
                                                       const Decision<T> *dec, const int64_t *cleft_off) {
-    __shared__ int64_t lds[BT / WAVE + 1];
-    if ((int64_t)blockIdx.x >= *nchunks) return;
-    const Chunk ch = chunks[blockIdx.x];
+    const int64_t ci = (int64_t)blockIdx.x * CPB + threadIdx.x / WAVE;
+    if (ci >= *nchunks) return;  // whole wave
+    const int lane = threadIdx.x & (WAVE - 1);
+    const uint64_t below = (1ull << lane) - 1;  // lanes before this one
+    const Chunk ch = chunks[ci];
     const Decision<T> dc = dec[ch.act];
     const int64_t nstart = range[act[ch.act]].start;
     const int64_t first = chunk_base[ch.act];
-    const int64_t left_before = cleft_off[blockIdx.x] - cleft_off[first];
+    const int64_t left_base = nstart + (cleft_off[ci] - cleft_off[first]);
     const int64_t n_left = cleft_off[first + nch[ch.act]] - cleft_off[first];
-    const int64_t right_before = (ch.begin - nstart) - left_before;
+    const int64_t right_base = nstart + n_left + (ch.begin - nstart) - (cleft_off[ci] - cleft_off[first]);
     const T *col = P + (int64_t)dc.axis * n;
-    const int64_t p0 = ch.begin + (int64_t)threadIdx.x * IPT;
-    bool f[IPT];
-    int64_t c = 0;
+    int64_t lacc = 0, racc = 0;
+#pragma unroll 2
+    for (int64_t base = ch.begin; base < ch.end; base += WAVE) {
+        const int64_t p = base + lane;
+        const bool in = p < ch.end;
+        const int64_t q = in ? p : base;  // a valid row for the idle lanes' loads
+        const bool f = in && goes_left(dc, col[q], p);
+        const uint64_t m = __ballot(f);
+        const int rank = __popcll(m & below);
+        const int nl = __popcll(m);
+        const int cnt = ch.end - base < WAVE ? (int)(ch.end - base) : WAVE;
+        if (in) {
+            const int64_t dst = f ? left_base + lacc + rank : right_base + racc + (lane - rank);
 #pragma unroll
-    for (int j = 0; j < IPT; j++) {
-        const int64_t p = p0 + j;
-        f[j] = p < ch.end && goes_left(dc, col[p], p);
-        c += f[j] ? 1 : 0;
-    }
-    int64_t tot;
-    int64_t lr = block_exclusive_scan<int64_t, BT>(c, lds, &tot);
-#pragma unroll
-    for (int j = 0; j < IPT; j++) {
-        const int64_t p = p0 + j;
-        if (p >= ch.end) break;
-        const int64_t local = p - ch.begin;
-        int64_t dst;
-        if (f[j]) {
-            dst = nstart + left_before + lr;
-            lr++;
-        } else {
-            dst = nstart + n_left + right_before + (local - lr);
+            for (int d = 0; d < D; d++) P2[(int64_t)d * n + dst] = P[(int64_t)d * n + p];
+            I2[dst] = I[p];
         }
-#pragma unroll
-        for (int d = 0; d < D; d++) P2[(int64_t)d * n + dst] = P[(int64_t)d * n + p];
-        I2[dst] = I[p];
+        lacc += nl;
+        racc += cnt - nl;
     }
 }
 
@@ -797,14 +819,15 @@ int build_impl(ckd_tree *t, const T *in, hipStream_t s) {
             if ((rc = scan_i64(d_nch, d_cb, nact, true, scratch.p, s))) break;
             k_chunk_fill<<<(unsigned)nact, 64, 0, s>>>(actc, t->range, d_cb, chunks.as<Chunk>());
             const int64_t *d_ntot = d_cb + nact;  // total chunks (device)
-            k_chunk_stats<T, D><<<(unsigned)nub, BT, 0, s>>>(chunks.as<Chunk>(), d_ntot, Pc, n, cstat.as<Ext<T>>());
-            k_node_decide<T, D><<<(unsigned)nact, BT, 0, s>>>(actc, d_cb, d_nch, cstat.as<Ext<T>>(),
+            const unsigned gch = (unsigned)ceil_div(nub, CPB);  // one wave per chunk slot
+            k_chunk_stats<T, D><<<gch, BT, 0, s>>>(chunks.as<Chunk>(), d_ntot, Pc, n, cstat.as<Ext<T>>());
+            k_node_decide<T, D><<<(unsigned)ceil_div(nact, CPB), BT, 0, s>>>(actc, nact, d_cb, d_nch, cstat.as<Ext<T>>(),
                                                               (NodeRec<T> *)t->nodes, (T *)t->bbox, (T *)t->dbox,
                                                               dec.as<Decision<T>>(), level == 0 && !t->periodic);
-            k_chunk_countleft<T><<<(unsigned)nub, BT, 0, s>>>(chunks.as<Chunk>(), d_ntot, nub, Pc, n,
+            k_chunk_countleft<T><<<gch, BT, 0, s>>>(chunks.as<Chunk>(), d_ntot, nub, Pc, n,
                                                                dec.as<Decision<T>>(), cleft.as<int64_t>());
             if ((rc = scan_i64(cleft.as<int64_t>(), cleft_off.as<int64_t>(), nub, true, scratch.p, s))) break;
-            k_chunk_scatter<T, D, IdxT><<<(unsigned)nub, BT, 0, s>>>(
+            k_chunk_scatter<T, D, IdxT><<<gch, BT, 0, s>>>(
                 chunks.as<Chunk>(), d_ntot, d_cb, d_nch, actc, t->range, Pc, Ic, Pn, In, n, dec.as<Decision<T>>(),
                 cleft_off.as<int64_t>());
             k_make_children<T, D><<<ga, 256, 0, s>>>(actc, nact, dec.as<Decision<T>>(), d_cb, d_nch,
