@@ -219,9 +219,11 @@ def _tree(tree) -> KDTree:
     return t
 
 
-def _build(points, leaf: int, boxsize, keep_points: bool = False) -> KDTree:
+def _build(points, leaf: int, boxsize, keep_points: bool = False, idx_bits: int = 0) -> KDTree:
     """Build from a torch tensor or array-like; boxsize None = non-periodic.
-    keep_points keeps the device copy of the input on the handle (.points)."""
+    keep_points keeps the device copy of the input on the handle (.points);
+    idx_bits 32 or 64 fixes the stored index width (0: 32 below 2^31
+    points, the reference's LONGINT=1 is 64)."""
     _require_gpu()
     if isinstance(points, torch.Tensor):
         dt = torch.float32 if points.dtype == torch.float32 else torch.float64
@@ -240,7 +242,7 @@ def _build(points, leaf: int, boxsize, keep_points: bool = False) -> KDTree:
         bs = (C.c_double * d)(*[float(x) for x in boxsize])
     out = C.c_void_p()
     prec = 4 if dt == torch.float32 else 8
-    check(lib.ckd_build(_vp(pts), n, d, prec, int(leaf), bs, 0, _stream(), C.byref(out)), "build_kdtree")
+    check(lib.ckd_build(_vp(pts), n, d, prec, int(leaf), bs, int(idx_bits), _stream(), C.byref(out)), "build_kdtree")
     tree = KDTree(out.value, n=n, ndim=d, leafsize=int(leaf), torch_dtype=dt,
                   np_dtype=np.float32 if prec == 4 else np.float64, device=pts.device,
                   periodic=boxsize is not None,
