@@ -117,7 +117,7 @@ int ckd_batch_plan(const ckd_tree *t, const int64_t *qoff, const int32_t *k, con
     if (e == hipSuccess) e = up((void **)&b->counters, nullptr, sizeof(int64_t) * 4);
     if (e == hipSuccess) e = up((void **)&b->counts, nullptr, sizeof(int64_t) * (b->nq > 0 ? b->nq : 1));
     if (e == hipSuccess) e = up(&b->scan_tmp, nullptr, scan_tmp_bytes(b->nq));
-    if (b->has_knn) {  // leaf ordering of the stack kNN path
+    if (b->nq > 0) {  // leaf ordering of the queries (stack kNN walk, query order of the other walks)
         const size_t nqb = (size_t)(b->nq > 0 ? b->nq : 1);
         if (e == hipSuccess) e = up((void **)&b->leaf_of, nullptr, sizeof(int32_t) * nqb);
         if (e == hipSuccess) e = up((void **)&b->rank, nullptr, sizeof(int32_t) * nqb);

@@ -147,7 +147,8 @@ __global__ __launch_bounds__(NT, 3) void k_batch(TreeView<T> tv, const int64_t *
                                               const int64_t *knn_off, int sorted, T *kdist, int64_t *kidx,
                                               int64_t *ball_counts, const int64_t *ball_off, T *bdist,
                                               int64_t *bidx, int64_t capacity, int64_t *overflow,
-                                              const int32_t *order, const int32_t *leaf_of, int Grt) {
+                                              const int32_t *order, const int32_t *leaf_of, int use_stack,
+                                              int Grt) {
     const int G = GC ? GC : Grt;
     __shared__ int64_t s_item;
     constexpr int CBMAX = sizeof(T) == 4 ? 16 : 8;
@@ -161,7 +162,7 @@ __global__ __launch_bounds__(NT, 3) void k_batch(TreeView<T> tv, const int64_t *
         __syncthreads();
         if (it >= nitems) break;
         const BatchItem w = items[it];
-        if (w.kind == 0 && order) {  // stack walk, whole waves
+        if (w.kind == 0 && use_stack) {  // stack walk, whole waves
             const int k = kset[w.set];
             StackEntry<T, D> *ws = stk[threadIdx.x / WAVE];
             const int64_t base = set_off[w.set], kb = knn_off[w.set], q0s = qoff[w.set];
@@ -180,7 +181,9 @@ __global__ __launch_bounds__(NT, 3) void k_batch(TreeView<T> tv, const int64_t *
             continue;
         }
         if ((int)threadIdx.x >= w.nq) continue;
-        const int64_t qg = w.q0 + threadIdx.x;
+        // one walk per query, the item's queries taken in leaf order so
+        // that neighbouring lanes walk the same nodes
+        const int64_t qg = order ? (int64_t)order[w.q0 + threadIdx.x] : w.q0 + threadIdx.x;
         T q[D];
 #pragma unroll
         for (int d = 0; d < D; d++) q[d] = queries[qg * D + d];
@@ -191,7 +194,7 @@ __global__ __launch_bounds__(NT, 3) void k_batch(TreeView<T> tv, const int64_t *
             const int64_t local = qg - qoff[w.set];
             T *drow = kdist + knn_off[w.set] + local * k;
             int64_t *irow = kidx + knn_off[w.set] + local * k;
-            const int32_t leaf = find_leaf<T, D>(tv, q, root);
+            const int32_t leaf = leaf_of ? leaf_of[qg] : find_leaf<T, D>(tv, q, root);
             Counters c;
             if (k <= 8) knn_one<T, D, PER, 8>(tv, q, leaf, k, sorted, T(FLT_MAX), drow, irow, base, c);
             else if (k <= 16) knn_one<T, D, PER, 16>(tv, q, leaf, k, sorted, T(FLT_MAX), drow, irow, base, c);
@@ -217,8 +220,11 @@ int run_t(ckd_batch *b, const T *queries, T *kdist, int64_t *kidx, int64_t *ball
     const ckd_tree *t = b->tree;
     const TreeView<T> tv = view<T>(t);
     k_batch_reset<<<1, 1, 0, s>>>(b->counters);
+    // leaf order of each set's queries: the stack kNN walk needs it, the
+    // per-query walks take their queries in it
     const int32_t *order = nullptr;
-    if (b->has_knn && b->knn_kernel == CKD_KNN_STACK && b->nq > 0) {
+    const int use_stack = b->has_knn && b->knn_kernel == CKD_KNN_STACK ? 1 : 0;
+    if (b->nq > 0) {
         const unsigned gq = (unsigned)ceil_div(b->nq, 256), gn = (unsigned)ceil_div(t->nnodes, 256);
         k_bfill<<<gn, 256, 0, s>>>(b->leaf_cnt, t->nnodes, 0);
         k_bfill<<<(unsigned)ceil_div(b->nsets, 256), 256, 0, s>>>(b->setpos, b->nsets, INT64_MAX);
@@ -232,16 +238,16 @@ int run_t(ckd_batch *b, const T *queries, T *kdist, int64_t *kidx, int64_t *ball
     }
     const int G = WAVE / b->qpw;
     if (b->n_a > 0) {
-        if (G == 1 || !order)
+        if (G == 1 || !use_stack)
             k_batch<T, D, PER, NT, SD, 1><<<b->grid, NT, 0, s>>>(tv, t->set_off, queries, b->items_a, b->n_a,
                                                                  b->counters, b->kset, b->rset, b->qoff, b->knn_off,
                                                                  b->sorted, kdist, kidx, b->counts, ball_off, bdist,
-                                                                 bidx, b->capacity, b->counters + 2, order, b->leaf_of, 1);
+                                                                 bidx, b->capacity, b->counters + 2, order, b->leaf_of, use_stack, 1);
         else
             k_batch<T, D, PER, NT, SD, 0><<<b->grid, NT, 0, s>>>(tv, t->set_off, queries, b->items_a, b->n_a,
                                                                  b->counters, b->kset, b->rset, b->qoff, b->knn_off,
                                                                  b->sorted, kdist, kidx, b->counts, ball_off, bdist,
-                                                                 bidx, b->capacity, b->counters + 2, order, b->leaf_of, G);
+                                                                 bidx, b->capacity, b->counters + 2, order, b->leaf_of, use_stack, G);
     }
     CKD_HIP(hipGetLastError());
     if (b->has_ball) {
@@ -252,7 +258,7 @@ int run_t(ckd_batch *b, const T *queries, T *kdist, int64_t *kidx, int64_t *ball
                                                                  b->counters + 1, b->kset, b->rset, b->qoff,
                                                                  b->knn_off, b->sorted, kdist, kidx, b->counts,
                                                                  ball_off, bdist, bidx, b->capacity, b->counters + 2,
-                                                                 nullptr, nullptr, 1);
+                                                                 order, b->leaf_of, 0, 1);
         CKD_HIP(hipGetLastError());
     }
     return CKD_OK;
