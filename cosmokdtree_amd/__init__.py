@@ -12,7 +12,7 @@ _API = ("KDTree", "ball_search", "box_search", "box_search_batch", "build_kdtree
         "knn_search")
 
 
-_MODULES = ("kdtree", "learned", "batch", "synthetic", "build")
+_MODULES = ("kdtree", "learned", "batch", "synthetic", "build", "autotune", "largebox", "pycosmokdtree")
 
 
 def __getattr__(name):
