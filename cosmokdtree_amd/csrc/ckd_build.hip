@@ -294,7 +294,6 @@ __global__ __launch_bounds__(BT) void k_chunk_scatter(const Chunk *chunks, const
     const int64_t right_base = nstart + n_left + (ch.begin - nstart) - (cleft_off[ci] - cleft_off[first]);
     const T *col = P + (int64_t)dc.axis * n;
     int64_t lacc = 0, racc = 0;
-#pragma unroll 2
     for (int64_t base = ch.begin; base < ch.end; base += WAVE) {
         const int64_t p = base + lane;
         const bool in = p < ch.end;
