@@ -219,7 +219,7 @@ __global__ __launch_bounds__(QT, 3) void k_knn_s(TreeView<T> tv, int2 root, cons
 
 // SPH smoothing length and density of each query, fused after the stack
 // walk (sph_row): writes h[qi], rho[qi] instead of the k-row.
-template <typename T, bool PER, int KMAX, int SD>
+template <typename T, bool PER, int KMAX, int SD, bool NOIDX>
 __global__ __launch_bounds__(QT, 3) void k_sph_s(TreeView<T> tv, int2 root, const T *queries, int64_t nq, int k,
                                               const int32_t *order, const int32_t *leaf_of, const T *mass,
                                               T *hout, T *rhoout) {
@@ -240,7 +240,7 @@ __global__ __launch_bounds__(QT, 3) void k_sph_s(TreeView<T> tv, int2 root, cons
     int found = 0;
     int32_t leaf = -1;
     if (active && root.y < 0) leaf = leaf_of ? leaf_of[qi] : find_leaf<T, D>(tv, q, 0);
-    knn_stack<T, D, PER, KMAX, SD, CB, 1>(tv, root, q, active, k, 1, T(FLT_MAX), hout + qi, nullptr, 0, c, buf_d,
+    knn_stack<T, D, PER, KMAX, SD, CB, 1, NOIDX>(tv, root, q, active, k, 1, T(FLT_MAX), hout + qi, nullptr, 0, c, buf_d,
                                          buf_s, threadIdx.x, QT, stk[threadIdx.x / WAVE], found, 1, 0, 0, leaf, mass,
                                          rhoout + qi);
 }
@@ -662,10 +662,15 @@ int sph_launch(const ckd_tree *t, const T *q, int64_t nq, int k, const T *mass, 
     LaunchTimer tm{t};
     int rc = tm.start(s);
     if (rc) return rc;
-#define CKD_SPH(KM)                                                                                        \
-    do {                                                                                                  \
-        if (t->depth <= 32) k_sph_s<T, PER, KM, 32><<<g, QT, 0, s>>>(tv, root, q, nq, k, order, leaf_of, mass, h, rho); \
-        else k_sph_s<T, PER, KM, 64><<<g, QT, 0, s>>>(tv, root, q, nq, k, order, leaf_of, mass, h, rho);              \
+#define CKD_SPH(KM)                                                                                          \
+    do {                                                                                                    \
+        if (mass) {                                                                                         \
+            if (t->depth <= 32) k_sph_s<T, PER, KM, 32, false><<<g, QT, 0, s>>>(tv, root, q, nq, k, order, leaf_of, mass, h, rho); \
+            else k_sph_s<T, PER, KM, 64, false><<<g, QT, 0, s>>>(tv, root, q, nq, k, order, leaf_of, mass, h, rho);          \
+        } else {  /* unit masses: the lists carry no rows */                                                \
+            if (t->depth <= 32) k_sph_s<T, PER, KM, 32, true><<<g, QT, 0, s>>>(tv, root, q, nq, k, order, leaf_of, mass, h, rho); \
+            else k_sph_s<T, PER, KM, 64, true><<<g, QT, 0, s>>>(tv, root, q, nq, k, order, leaf_of, mass, h, rho);           \
+        }                                                                                                   \
     } while (0)
     if (k <= 8) CKD_SPH(8);
     else if (k <= 16) CKD_SPH(16);

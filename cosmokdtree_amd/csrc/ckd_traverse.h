@@ -71,6 +71,22 @@ __device__ __forceinline__ void list_insert(T (&bd)[KMAX], int32_t (&bs)[KMAX], 
     bd[0] = minv(bd[0], d);
 }
 
+// list_insert without the rows: slot j becomes med3(old j-1, d, old j), one
+// instruction per slot (callers that need only the distances).
+template <typename T, int KMAX>
+__device__ __forceinline__ void list_insert_d(T (&bd)[KMAX], T d) {
+#pragma unroll
+    for (int j = KMAX - 1; j > 0; --j) bd[j] = med3v(bd[j - 1], d, bd[j]);
+    bd[0] = minv(bd[0], d);
+}
+
+// Insert into the list, with its rows unless NOIDX.
+template <bool NOIDX, typename T, int KMAX>
+__device__ __forceinline__ void list_put(T (&bd)[KMAX], int32_t (&bs)[KMAX], T d, int32_t s) {
+    if constexpr (NOIDX) list_insert_d<T, KMAX>(bd, d);
+    else list_insert<T, KMAX>(bd, bs, d, s);
+}
+
 // The leaf a point falls in below `root`: the nearer child at every level
 // (the first descent of knn_search_recursive_hyperp, cosmokdtree.f90:820-836).
 template <typename T, int D>
@@ -414,7 +430,7 @@ __device__ __forceinline__ void load_rows(const T *pts, int32_t p, T (&x)[PB][D]
 // when its rows lie in the lane's prefilled range [xs, xs + xm) (knn_stack):
 // a leaf's rows are either all inside an ancestor's row range or all
 // outside it.
-template <typename T, int D, bool PER, int KMAX, int CB>
+template <typename T, int D, bool PER, int KMAX, int CB, bool NOIDX = false>
 __device__ __forceinline__ void wave_scan_rows(const TreeView<T> &tv, int32_t pb, int32_t pe, const T (&q)[D],
                                                bool run_q, T (&bd)[KMAX], int32_t (&bs)[KMAX], Counters &c,
                                                T *buf_d, int32_t *buf_s, int tid, int nthreads, int32_t xs,
@@ -441,7 +457,7 @@ __device__ __forceinline__ void wave_scan_rows(const TreeView<T> &tv, int32_t pb
                     const T d = dist2<T, D, PER>(x[j], q, tv.L);
                     if (run && p + j < ce && d < bound) {
                         *wd = d;
-                        *ws = p + j;
+                        if constexpr (!NOIDX) *ws = p + j;
                         wd += nthreads;
                         ws += nthreads;
                         cnt++;
@@ -454,7 +470,7 @@ __device__ __forceinline__ void wave_scan_rows(const TreeView<T> &tv, int32_t pb
                 if (j < cnt) {
                     const T d = buf_d[j * nthreads + tid];
                     if (d < bd[KMAX - 1]) {
-                        list_insert<T, KMAX>(bd, bs, d, buf_s[j * nthreads + tid]);
+                        list_put<NOIDX, T, KMAX>(bd, bs, d, NOIDX ? 0 : buf_s[j * nthreads + tid]);
                         c.inserted++;
                     }
                 }
@@ -469,7 +485,7 @@ __device__ __forceinline__ void wave_scan_rows(const TreeView<T> &tv, int32_t pb
             for (int j = 0; j < PB; j++) {
                 const T d = dist2<T, D, PER>(x[j], q, tv.L);
                 if (run && p + j < pe && d < bd[KMAX - 1]) {
-                    list_insert<T, KMAX>(bd, bs, d, p + j);
+                    list_put<NOIDX, T, KMAX>(bd, bs, d, p + j);
                     c.inserted++;
                 }
             }
@@ -569,7 +585,7 @@ __device__ __forceinline__ T wrap_free_bound(const T (&q)[D], const T *L) {
 // Returns true when the traversal is complete; with CHECK it instead
 // returns false, (a, b) not yet processed, as soon as every running lane's
 // bound is within its wrap-free bound `safe2`.
-template <typename T, int D, bool PER, int KMAX, int SD, int CB, bool CHECK>
+template <typename T, int D, bool PER, int KMAX, int SD, int CB, bool CHECK, bool NOIDX = false>
 __device__ __forceinline__ bool stack_walk(const TreeView<T> &tv, const T (&q)[D], bool run, T safe2, T (&bd)[KMAX],
                                            int32_t (&bs)[KMAX], int32_t &a, int32_t &b, int &sp, Counters &c,
                                            T *buf_d, int32_t *buf_s, int tid, int nthreads, StackEntry<T, D> *stk,
@@ -580,7 +596,8 @@ __device__ __forceinline__ bool stack_walk(const TreeView<T> &tv, const T (&q)[D
         bool descend = false;
         if (b >= 0) {
             if (G == 1)
-                wave_scan_rows<T, D, PER, KMAX, CB>(tv, a, a + b, q, run, bd, bs, c, buf_d, buf_s, tid, nthreads, xs, xm);
+                wave_scan_rows<T, D, PER, KMAX, CB, NOIDX>(tv, a, a + b, q, run, bd, bs, c, buf_d, buf_s, tid, nthreads,
+                                                           xs, xm);
             else group_scan_rows<T, D, PER, KMAX>(tv, a, a + b, q, run, bd, bs, c, G, sub, gbase);
         } else {
             T b0[2 * D], b1[2 * D];
@@ -640,8 +657,10 @@ __device__ __forceinline__ bool stack_walk(const TreeView<T> &tv, const T (&q)[D
 // rho = sum_j m_j W(r_j, h), W(r, h) = 1/(pi h^3) w(r/h),
 // w(q) = 1 - 1.5 q^2 + 0.75 q^3 (q < 1), 0.25 (2 - q)^3 (q < 2), else 0
 // (Monaghan & Lattanzio 1985).  The query itself counts when it is a tree
-// point (r = 0).  mass: per original point, nullptr = unit masses.
-template <typename T, int KMAX>
+// point (r = 0).  mass: per original point, nullptr = unit masses.  With
+// NOIDX the list carries no rows (unit masses only); every live slot holds
+// a point since k <= n.
+template <typename T, int KMAX, bool NOIDX = false>
 __device__ __forceinline__ void sph_row(const TreeView<T> &tv, const T (&bd)[KMAX], const int32_t (&bs)[KMAX], int k,
                                         const T *mass, int64_t idx_base, T *h_out, T *rho_out) {
     const T h = sqrt_rn<T>(bd[KMAX - 1]) * T(0.5);
@@ -649,12 +668,12 @@ __device__ __forceinline__ void sph_row(const TreeView<T> &tv, const T (&bd)[KMA
     T rho = T(0);
 #pragma unroll
     for (int j = 0; j < KMAX; j++) {
-        if (j >= KMAX - k && bs[j] >= 0) {
+        if (j >= KMAX - k && (NOIDX || bs[j] >= 0)) {
             const T q = sqrt_rn<T>(bd[j]) * inv_h;
             const T tq = T(2) - q;
             const T w = q < T(1) ? T(1) - T(1.5) * q * q + T(0.75) * q * q * q
                                  : (q < T(2) ? T(0.25) * tq * tq * tq : T(0));
-            const T m = mass ? mass[load_idx(tv.idx, tv.idx64, bs[j]) - idx_base] : T(1);
+            const T m = !NOIDX && mass ? mass[load_idx(tv.idx, tv.idx64, bs[j]) - idx_base] : T(1);
             rho = rho + m * w;
         }
     }
@@ -678,7 +697,7 @@ __device__ __forceinline__ void sph_row(const TreeView<T> &tv, const T (&bd)[KMA
 // when the shared walk starts, which then skips those rows.  Far points
 // that the walk's first leaves would have inserted, only to be displaced,
 // are never inserted.
-template <typename T, int D, bool PER, int KMAX, int SD, int CB, int EPI = 0>
+template <typename T, int D, bool PER, int KMAX, int SD, int CB, int EPI = 0, bool NOIDX = false>
 __device__ __forceinline__ bool knn_stack(const TreeView<T> &tv, int2 root, const T (&q)[D], bool active, int k,
                                           int sorted, T seed2, T *drow, int64_t *irow, int64_t idx_base, Counters &c,
                                           T *buf_d, int32_t *buf_s, int tid, int nthreads, StackEntry<T, D> *stk,
@@ -734,19 +753,19 @@ __device__ __forceinline__ bool knn_stack(const TreeView<T> &tv, int2 root, cons
                 // which keeps the list in place instead of a selected copy
                 const T d = r + j < (int)xm ? dist2<T, D, PER>(x[j], q, tv.L) : T(INFINITY);
                 c.inserted += d < bd[KMAX - 1] ? 1 : 0;
-                list_insert<T, KMAX>(bd, bs, d, xs + r + j);
+                list_put<NOIDX, T, KMAX>(bd, bs, d, xs + r + j);
             }
         }
         c.tested += xm;
     }
     if constexpr (PER) {
         const T safe2 = wrap_free_bound<T, D>(q, tv.L);
-        if (!stack_walk<T, D, true, KMAX, SD, CB, true>(tv, q, run, safe2, bd, bs, a, b, sp, c, buf_d, buf_s, tid,
+        if (!stack_walk<T, D, true, KMAX, SD, CB, true, NOIDX>(tv, q, run, safe2, bd, bs, a, b, sp, c, buf_d, buf_s, tid,
                                                         nthreads, stk, G, sub, gbase, xs, xm))
-            stack_walk<T, D, false, KMAX, SD, CB, false>(tv, q, run, safe2, bd, bs, a, b, sp, c, buf_d, buf_s, tid,
+            stack_walk<T, D, false, KMAX, SD, CB, false, NOIDX>(tv, q, run, safe2, bd, bs, a, b, sp, c, buf_d, buf_s, tid,
                                                          nthreads, stk, G, sub, gbase, xs, xm);
     } else {
-        stack_walk<T, D, false, KMAX, SD, CB, false>(tv, q, run, T(0), bd, bs, a, b, sp, c, buf_d, buf_s, tid,
+        stack_walk<T, D, false, KMAX, SD, CB, false, NOIDX>(tv, q, run, T(0), bd, bs, a, b, sp, c, buf_d, buf_s, tid,
                                                      nthreads, stk, G, sub, gbase, xs, xm);
     }
     if (!active) return true;
@@ -759,7 +778,7 @@ __device__ __forceinline__ bool knn_stack(const TreeView<T> &tv, int2 root, cons
     }
     if (sub == 0) {
         if constexpr (EPI == 0) knn_write_row<T, KMAX>(tv, bd, bs, k, sorted, drow, irow, idx_base);
-        else sph_row<T, KMAX>(tv, bd, bs, k, mass, idx_base, drow, rho_out);  // EPI 1: h to *drow
+        else sph_row<T, KMAX, NOIDX>(tv, bd, bs, k, mass, idx_base, drow, rho_out);  // EPI 1: h to *drow
     }
     return true;
 }

@@ -10,6 +10,7 @@ change rho.
 """
 import numpy as np
 import pytest
+import torch
 
 from oracle import sph_oracle as so
 from oracle.oracle import OracleTree
@@ -101,3 +102,14 @@ def test_sph_rejects(ckd):
     finally:
         t2.free()
         t3.free()
+
+
+@pytest.mark.parametrize("k", [8, 32, 64])
+def test_sph_unit_masses_paths_agree(scase, k):
+    """Without masses the walk keeps distances only (no rows); with explicit
+    unit masses it keeps rows and looks the masses up.  The lists hold the
+    same distances in the same order, so h and rho agree bit for bit."""
+    q = scase["q"]
+    h0, r0 = scase["tree"].sph_density(q, k)
+    h1, r1 = scase["tree"].sph_density(q, k, np.ones(len(scase["pts"]), dtype=scase["pts"].dtype))
+    assert torch.equal(h0, h1) and torch.equal(r0, r1)
