@@ -28,6 +28,7 @@ EXPORTS = (
     "ckd_build", "ckd_build_forest", "ckd_free", "ckd_get_info", "ckd_knn", "ckd_ball_count", "ckd_ball_fill",
     "ckd_box_count", "ckd_box_fill", "ckd_tree_dump", "ckd_last_error", "ckd_version", "ckd_knn_config",
     "ckd_knn_timing", "ckd_knn_learned", "ckd_mlp_table", "ckd_tree_save", "ckd_tree_load", "ckd_sph_density",
+    "ckd_release_memory",
 )
 KNN_KERNELS = {"auto": -1, "lane": 0, "wave": 1, "stack": 2}
 
@@ -62,6 +63,7 @@ def _load():
     lib.ckd_box_count.argtypes = [vp, vp, i64, vp, C.POINTER(i64), vp]
     lib.ckd_box_fill.argtypes = [vp, vp, i64, vp, vp, vp]
     lib.ckd_tree_dump.argtypes = [vp] * 11
+    lib.ckd_release_memory.argtypes = []
     lib.ckd_last_error.restype = C.c_char_p
     lib.ckd_version.restype = C.c_char_p
     for name in EXPORTS:

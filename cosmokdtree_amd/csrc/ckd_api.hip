@@ -36,17 +36,15 @@ int hip_fail(hipError_t e, const char *what, const char *file, int line) {
 static int grow(void **buf, size_t *have, size_t bytes, void **p, hipStream_t s) {
     if (*have < bytes) {
         if (*buf) {
-            CKD_HIP(hipStreamSynchronize(s));
-            (void)hipFree(*buf);
-            *buf = nullptr;
-            *have = 0;
+            // the old buffer may still be read by a call on another stream
+            CKD_HIP(hipDeviceSynchronize());
+            dev_free(*buf, s);
         }
+        *buf = nullptr;
+        *have = 0;
         const size_t want = bytes + bytes / 4;
-        hipError_t e = hipMalloc(buf, want);
-        if (e != hipSuccess) {
-            *buf = nullptr;
-            return hip_fail(e, "workspace", __FILE__, __LINE__);
-        }
+        int rc = dev_alloc(buf, want, s);
+        if (rc) return rc;
         *have = want;
     }
     *p = *buf;
@@ -57,28 +55,28 @@ int workspace(ckd_tree *t, int slot, size_t bytes, void **p, hipStream_t s) {
     return grow(&t->ws[slot], &t->ws_bytes[slot], bytes, p, s);
 }
 
+// Frees every device array of the tree.  Like hipFree, it first waits for
+// all work on the tree's device, so no stream still reads the arrays.
 static void release(ckd_tree *t) {
     for (auto &ev : t->knn_events) {
         (void)hipEventDestroy(ev.first);
         (void)hipEventDestroy(ev.second);
     }
     t->knn_events.clear();
-    if (t->set_off) (void)hipFree(t->set_off);
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    if (cur != t->device) (void)hipSetDevice(t->device);
+    (void)hipDeviceSynchronize();
+    void *arrays[] = {t->set_off, t->ws[0], t->ws[1], t->ws[2], t->nodes, t->bbox, t->dbox, t->range,
+                      t->pts,     t->idx,   t->wide,  t->wroot, t->wrange};
+    static_assert(ckd_tree::NWS == 3, "workspace slots listed above");
+    for (void *a : arrays) dev_free_idle(a);
+    if (cur != t->device) (void)hipSetDevice(cur);
     t->set_off = nullptr;
     for (int i = 0; i < ckd_tree::NWS; i++) {
-        if (t->ws[i]) (void)hipFree(t->ws[i]);
         t->ws[i] = nullptr;
         t->ws_bytes[i] = 0;
     }
-    if (t->nodes) (void)hipFree(t->nodes);
-    if (t->bbox) (void)hipFree(t->bbox);
-    if (t->dbox) (void)hipFree(t->dbox);
-    if (t->range) (void)hipFree(t->range);
-    if (t->pts) (void)hipFree(t->pts);
-    if (t->idx) (void)hipFree(t->idx);
-    if (t->wide) (void)hipFree(t->wide);
-    if (t->wroot) (void)hipFree(t->wroot);
-    if (t->wrange) (void)hipFree(t->wrange);
     t->nodes = t->bbox = t->dbox = t->pts = t->idx = t->wide = nullptr;
     t->wroot = t->wrange = nullptr;
     t->nwide = 0;
@@ -90,18 +88,22 @@ static void release(ckd_tree *t) {
 static int counts_to_offsets(int64_t *offsets, int64_t n, int64_t *total, hipStream_t s) {
     int64_t *tmp = nullptr;
     void *scr = nullptr;
-    CKD_HIP(hipMalloc(&tmp, sizeof(int64_t) * (n > 0 ? n : 1)));
-    CKD_HIP(hipMalloc(&scr, scan_tmp_bytes(n)));
+    int rc = dev_alloc((void **)&tmp, sizeof(int64_t) * (n > 0 ? n : 1), s);
+    if (!rc) rc = dev_alloc(&scr, scan_tmp_bytes(n), s);
+    if (rc) {
+        dev_free(tmp, s);
+        return rc;
+    }
     if (n > 0) CKD_HIP(hipMemcpyAsync(tmp, offsets + 1, sizeof(int64_t) * n, hipMemcpyDeviceToDevice, s));
-    int rc = scan_i64(tmp, offsets, n, true, scr, s);
+    rc = scan_i64(tmp, offsets, n, true, scr, s);
     int64_t tot = 0;
     if (!rc) {
         hipError_t e = hipMemcpyAsync(&tot, offsets + n, sizeof(int64_t), hipMemcpyDeviceToHost, s);
         if (e == hipSuccess) e = hipStreamSynchronize(s);
         if (e != hipSuccess) rc = hip_fail(e, "offsets total", __FILE__, __LINE__);
     }
-    (void)hipFree(tmp);
-    (void)hipFree(scr);
+    dev_free(tmp, s);
+    dev_free(scr, s);
     if (total) *total = tot;
     return rc;
 }
@@ -185,16 +187,14 @@ static int build_common(const void *points, int64_t n, const int64_t *set_offset
     }
     (void)hipGetDevice(&t->device);
     hipStream_t s = (hipStream_t)stream;
-    hipError_t e = hipMalloc(&t->set_off, sizeof(int64_t) * (nsets + 1));
-    if (e == hipSuccess) {
-        if (set_offsets) {
-            e = hipMemcpy(t->set_off, set_offsets, sizeof(int64_t) * (nsets + 1), hipMemcpyHostToDevice);
-        } else {
-            const int64_t two[2] = {0, n};
-            e = hipMemcpy(t->set_off, two, sizeof(two), hipMemcpyHostToDevice);
-        }
+    int rc = dev_alloc((void **)&t->set_off, sizeof(int64_t) * (nsets + 1), s);
+    if (!rc) {
+        const int64_t two[2] = {0, n};
+        hipError_t e = hipMemcpyAsync(t->set_off, set_offsets ? set_offsets : two, sizeof(int64_t) * (nsets + 1),
+                                      hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);  // the host array may go away on return
+        rc = e == hipSuccess ? build_tree(t, points, s) : hip_fail(e, "set offsets", __FILE__, __LINE__);
     }
-    int rc = e == hipSuccess ? build_tree(t, points, s) : hip_fail(e, "set offsets", __FILE__, __LINE__);
     if (rc) {
         release(t);
         delete t;

@@ -32,6 +32,7 @@ namespace {
 constexpr int CH = 2048;          // points per chunk
 constexpr int BT = 256;           // threads per workgroup
 constexpr int CPB = BT / WAVE;    // chunks (or nodes) per workgroup, one per wave
+constexpr int64_t NODE_THREAD_MIN_CH = 4;  // decide with one thread per node when chunks <= 4 x nodes
 
 struct Chunk {
     int64_t begin, end;  // positions in the working buffers
@@ -177,25 +178,11 @@ __global__ __launch_bounds__(BT) void k_chunk_stats(const Chunk *chunks, const i
         for (int d = 0; d < D; d++) cstat[ci * D + d] = e[d];
 }
 
-// One wave per active node: its lanes merge the node's chunk statistics,
-// lane 0 applies the split rule.
+// The split rule for active node a (id) from its merged extremes e:
+// longest cell side with the spread tie-break, midpoint, slide.
 template <typename T, int D>
-__global__ __launch_bounds__(BT) void k_node_decide(const int32_t *act, int64_t nact, const int64_t *chunk_base,
-                                                    const int64_t *nch, const Ext<T> *cstat, NodeRec<T> *nodes,
-                                                    T *bbox, T *dbox, Decision<T> *dec, int root_from_data) {
-    const int64_t a = (int64_t)blockIdx.x * CPB + threadIdx.x / WAVE;
-    if (a >= nact) return;  // whole wave
-    const int lane = threadIdx.x & (WAVE - 1);
-    const int32_t id = act[a];
-    Ext<T> e[D];
-#pragma unroll
-    for (int d = 0; d < D; d++) e[d] = ext_empty<T>();
-    const int64_t c0 = chunk_base[a], c1 = c0 + nch[a];
-    for (int64_t c = c0 + lane; c < c1; c += WAVE)
-#pragma unroll
-        for (int d = 0; d < D; d++) ext_merge(e[d], cstat[c * D + d]);
-    wave_reduce_ext<T, D>(e);
-    if (lane != 0) return;
+__device__ __forceinline__ void decide_node(int64_t a, int32_t id, const Ext<T> (&e)[D], NodeRec<T> *nodes, T *bbox,
+                                            T *dbox, Decision<T> *dec, int root_from_data) {
     T *bb = bbox + (int64_t)id * 2 * D;
     T *db = dbox + (int64_t)id * 2 * D;
 #pragma unroll
@@ -242,7 +229,44 @@ __global__ __launch_bounds__(BT) void k_node_decide(const int32_t *act, int64_t 
     }
     dec[a] = dc;
     nodes[id].split = dc.mid;
-    nodes[id].axis = axis;
+    nodes[id].axis = axis;}
+
+// One wave per active node: its lanes merge the node's chunk statistics,
+// lane 0 applies the split rule (levels of few nodes with many chunks).
+template <typename T, int D>
+__global__ __launch_bounds__(BT) void k_node_decide(const int32_t *act, int64_t nact, const int64_t *chunk_base,
+                                                    const int64_t *nch, const Ext<T> *cstat, NodeRec<T> *nodes,
+                                                    T *bbox, T *dbox, Decision<T> *dec, int root_from_data) {
+    const int64_t a = (int64_t)blockIdx.x * CPB + threadIdx.x / WAVE;
+    if (a >= nact) return;  // whole wave
+    const int lane = threadIdx.x & (WAVE - 1);
+    Ext<T> e[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) e[d] = ext_empty<T>();
+    const int64_t c0 = chunk_base[a], c1 = c0 + nch[a];
+    for (int64_t c = c0 + lane; c < c1; c += WAVE)
+#pragma unroll
+        for (int d = 0; d < D; d++) ext_merge(e[d], cstat[c * D + d]);
+    wave_reduce_ext<T, D>(e);
+    if (lane == 0) decide_node<T, D>(a, act[a], e, nodes, bbox, dbox, dec, root_from_data);
+}
+
+// One thread per active node, merging its chunks in order (levels of many
+// nodes with one or a few chunks each).
+template <typename T, int D>
+__global__ __launch_bounds__(BT) void k_node_decide_t(const int32_t *act, int64_t nact, const int64_t *chunk_base,
+                                                      const int64_t *nch, const Ext<T> *cstat, NodeRec<T> *nodes,
+                                                      T *bbox, T *dbox, Decision<T> *dec, int root_from_data) {
+    const int64_t a = (int64_t)blockIdx.x * BT + threadIdx.x;
+    if (a >= nact) return;
+    Ext<T> e[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) e[d] = ext_empty<T>();
+    const int64_t c0 = chunk_base[a], c1 = c0 + nch[a];
+    for (int64_t c = c0; c < c1; c++)
+#pragma unroll
+        for (int d = 0; d < D; d++) ext_merge(e[d], cstat[c * D + d]);
+    decide_node<T, D>(a, act[a], e, nodes, bbox, dbox, dec, root_from_data);
 }
 
 // Left count of each chunk (one wave per chunk, 64 points per ballot);
@@ -582,31 +606,38 @@ __global__ void k_check_range(const T *in, int64_t n, TreeView<T> tv, int *bad) 
 
 // ------------------------------------------------------------ host driver
 
+// Scratch device buffer from the library's cache, handed back on its stream
+// (after the work enqueued there) when it goes out of scope.
 struct DevBuf {
+    hipStream_t s;
     void *p = nullptr;
-    ~DevBuf() {
-        if (p) (void)hipFree(p);
-    }
+    explicit DevBuf(hipStream_t stream) : s(stream) {}
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { dev_free(p, s); }
     int alloc(size_t bytes) {
-        if (p) (void)hipFree(p);
+        dev_free(p, s);
         p = nullptr;
-        hipError_t e = hipMalloc(&p, bytes ? bytes : 16);
-        if (e != hipSuccess) {
-            p = nullptr;
-            set_error("device allocation of %zu bytes failed: %s", bytes, hipGetErrorString(e));
-            return CKD_ERR_MEMORY;
-        }
-        return CKD_OK;
+        return dev_alloc(&p, bytes, s);
     }
     template <typename X>
     X *as() const { return (X *)p; }
 };
 
+// One pinned int64 per host thread for the per-level count read-back,
+// allocated on first use and kept until the process ends.
+int pinned_slot(int64_t **p) {
+    static thread_local int64_t *slot = nullptr;
+    if (!slot) CKD_HIP(hipHostMalloc((void **)&slot, sizeof(int64_t), hipHostMallocDefault));
+    *p = slot;
+    return CKD_OK;
+}
+
 template <typename T, int D>
 int renumber_dfs(ckd_tree *t, const std::vector<int64_t> &lvl_start, hipStream_t s) {
     const int64_t nn = t->nnodes, nsets = t->nsets;
     const int nlev = (int)lvl_start.size() - 1;
-    DevBuf desc, newid, pairbase, rdesc, roff, scr;
+    DevBuf desc(s), newid(s), pairbase(s), rdesc(s), roff(s), scr(s);
     int rc;
     if ((rc = desc.alloc(sizeof(int64_t) * nn)) || (rc = newid.alloc(sizeof(int64_t) * nn)) ||
         (rc = pairbase.alloc(sizeof(int64_t) * nn)) || (rc = rdesc.alloc(sizeof(int64_t) * nsets)) ||
@@ -630,19 +661,17 @@ int renumber_dfs(ckd_tree *t, const std::vector<int64_t> &lvl_start, hipStream_t
                                                                             pairbase.as<int64_t>());
     }
     void *n2 = nullptr, *b2 = nullptr, *d2 = nullptr, *r2 = nullptr;
-    CKD_HIP(hipMalloc(&n2, sizeof(NodeRec<T>) * nn));
-    CKD_HIP(hipMalloc(&b2, sizeof(T) * 2 * D * nn));
-    CKD_HIP(hipMalloc(&d2, sizeof(T) * 2 * D * nn));
-    CKD_HIP(hipMalloc(&r2, sizeof(Range) * nn));
+    if ((rc = dev_alloc(&n2, sizeof(NodeRec<T>) * nn, s)) || (rc = dev_alloc(&b2, sizeof(T) * 2 * D * nn, s)) ||
+        (rc = dev_alloc(&d2, sizeof(T) * 2 * D * nn, s)) || (rc = dev_alloc(&r2, sizeof(Range) * nn, s))) {
+        for (void *x : {n2, b2, d2, r2}) dev_free(x, s);
+        return rc;
+    }
     k_permute_nodes<T, D><<<(unsigned)ceil_div(nn, 256), 256, 0, s>>>(
         nodes, (const T *)t->bbox, (const T *)t->dbox, t->range, nn, newid.as<int64_t>(), (NodeRec<T> *)n2, (T *)b2,
         (T *)d2, (Range *)r2);
     CKD_HIP(hipGetLastError());
-    CKD_HIP(hipStreamSynchronize(s));
-    (void)hipFree(t->nodes);
-    (void)hipFree(t->bbox);
-    (void)hipFree(t->dbox);
-    (void)hipFree(t->range);
+    // the level-order arrays go once the permutation has read them
+    for (void *x : {t->nodes, t->bbox, t->dbox, (void *)t->range}) dev_free(x, s);
     t->nodes = n2;
     t->bbox = b2;
     t->dbox = d2;
@@ -656,7 +685,7 @@ template <typename T, int D>
 int build_wide(ckd_tree *t, hipStream_t s) {
     const int64_t nn = t->nnodes, nsets = t->nsets;
     if (t->n >= INT32_MAX) return CKD_OK;  // stack traversals not offered
-    DevBuf flag, widx, scr;
+    DevBuf flag(s), widx(s), scr(s);
     int rc;
     if ((rc = flag.alloc(sizeof(int64_t) * nn)) || (rc = widx.alloc(sizeof(int64_t) * (nn + 1))) ||
         (rc = scr.alloc(scan_tmp_bytes(nn))))
@@ -668,16 +697,16 @@ int build_wide(ckd_tree *t, hipStream_t s) {
     int64_t ninner = 0;
     CKD_HIP(hipMemcpyAsync(&ninner, widx.as<int64_t>() + nn, sizeof(int64_t), hipMemcpyDeviceToHost, s));
     CKD_HIP(hipStreamSynchronize(s));
-    CKD_HIP(hipMalloc(&t->wide, wide_bytes<T, D>() * (size_t)(ninner > 0 ? ninner : 1)));
-    CKD_HIP(hipMalloc((void **)&t->wroot, sizeof(int2) * (size_t)nsets));
-    CKD_HIP(hipMalloc((void **)&t->wrange, sizeof(int2) * (size_t)(ninner > 0 ? ninner : 1)));
+    if ((rc = dev_alloc(&t->wide, wide_bytes<T, D>() * (size_t)(ninner > 0 ? ninner : 1), s)) ||
+        (rc = dev_alloc((void **)&t->wroot, sizeof(int2) * (size_t)nsets, s)) ||
+        (rc = dev_alloc((void **)&t->wrange, sizeof(int2) * (size_t)(ninner > 0 ? ninner : 1), s)))
+        return rc;
     t->nwide = ninner;
     k_fill_wide<T, D><<<g, 256, 0, s>>>(nodes, (const T *)t->dbox, t->range, widx.as<int64_t>(), nn,
                                         (char *)t->wide, t->wrange);
     k_root_pairs<T><<<(unsigned)ceil_div(nsets, 256), 256, 0, s>>>(nodes, t->range, widx.as<int64_t>(), nsets,
                                                                     t->wroot);
     CKD_HIP(hipGetLastError());
-    CKD_HIP(hipStreamSynchronize(s));  // the scratch buffers are freed on return
     return CKD_OK;
 }
 
@@ -686,36 +715,45 @@ int build_impl(ckd_tree *t, const T *in, hipStream_t s) {
     const int64_t n = t->n;
     const int leaf = t->leafsize;
     const TreeView<T> tvL = view<T>(t);
-    hipEvent_t ev0, ev1;
-    CKD_HIP(hipEventCreate(&ev0));
-    CKD_HIP(hipEventCreate(&ev1));
+    struct Events {  // destroyed on every return
+        hipEvent_t a = nullptr, b = nullptr;
+        ~Events() {
+            if (a) (void)hipEventDestroy(a);
+            if (b) (void)hipEventDestroy(b);
+        }
+    } ev;
+    CKD_HIP(hipEventCreate(&ev.a));
+    CKD_HIP(hipEventCreate(&ev.b));
+    hipEvent_t ev0 = ev.a, ev1 = ev.b;
     CKD_HIP(hipEventRecord(ev0, s));
 
     if (t->periodic) {
-        DevBuf bad;
+        DevBuf bad(s);
         int rc = bad.alloc(sizeof(int));
         if (rc) return rc;
         CKD_HIP(hipMemsetAsync(bad.p, 0, sizeof(int), s));
         k_check_range<T, D><<<(unsigned)ceil_div(n, 256), 256, 0, s>>>(in, n, tvL, bad.as<int>());
-        int hbad = 0;
-        CKD_HIP(hipMemcpyAsync(&hbad, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        int64_t *hbad = nullptr;
+        if ((rc = pinned_slot(&hbad))) return rc;
+        *hbad = 0;
+        CKD_HIP(hipMemcpyAsync(hbad, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
         CKD_HIP(hipStreamSynchronize(s));
-        if (hbad) {
+        if (*hbad) {
             set_error("Points outside (-L/2, L/2) range !!");
             return CKD_ERR_RANGE;
         }
     }
 
     constexpr int PS = pstride(D);
-    DevBuf P, P2, I, I2;
+    DevBuf P(s), P2(s), I(s), I2(s);
     int rc;
     if ((rc = P.alloc(sizeof(T) * D * n)) || (rc = P2.alloc(sizeof(T) * D * n)) ||
         (rc = I.alloc(sizeof(IdxT) * n)) || (rc = I2.alloc(sizeof(IdxT) * n)))
         return rc;
     // final arrays, owned by the tree
-    CKD_HIP(hipMalloc(&t->pts, sizeof(T) * PS * (n + PTS_PAD)));
+    if ((rc = dev_alloc(&t->pts, sizeof(T) * PS * (n + PTS_PAD), s))) return rc;
     CKD_HIP(hipMemsetAsync((T *)t->pts + (size_t)PS * n, 0, sizeof(T) * PS * PTS_PAD, s));
-    CKD_HIP(hipMalloc(&t->idx, sizeof(IdxT) * n));
+    if ((rc = dev_alloc(&t->idx, sizeof(IdxT) * n, s))) return rc;
 
     const int64_t nsets_ = t->nsets;
     int64_t cap = 4 * n / (leaf > 1 ? leaf : 1) + 64;
@@ -723,20 +761,18 @@ int build_impl(ckd_tree *t, const T *in, hipStream_t s) {
     if (cap < 2 * nsets_ + 64) cap = 2 * nsets_ + 64;
     auto alloc_nodes = [&](int64_t c) -> int {
         void *nn = nullptr, *nb = nullptr, *nd = nullptr, *nr = nullptr;
-        CKD_HIP(hipMalloc(&nn, sizeof(NodeRec<T>) * c));
-        CKD_HIP(hipMalloc(&nb, sizeof(T) * 2 * D * c));
-        CKD_HIP(hipMalloc(&nd, sizeof(T) * 2 * D * c));
-        CKD_HIP(hipMalloc(&nr, sizeof(Range) * c));
+        int e;
+        if ((e = dev_alloc(&nn, sizeof(NodeRec<T>) * c, s)) || (e = dev_alloc(&nb, sizeof(T) * 2 * D * c, s)) ||
+            (e = dev_alloc(&nd, sizeof(T) * 2 * D * c, s)) || (e = dev_alloc(&nr, sizeof(Range) * c, s))) {
+            for (void *x : {nn, nb, nd, nr}) dev_free(x, s);
+            return e;
+        }
         if (t->nodes) {
             CKD_HIP(hipMemcpyAsync(nn, t->nodes, sizeof(NodeRec<T>) * t->nnodes, hipMemcpyDeviceToDevice, s));
             CKD_HIP(hipMemcpyAsync(nb, t->bbox, sizeof(T) * 2 * D * t->nnodes, hipMemcpyDeviceToDevice, s));
             CKD_HIP(hipMemcpyAsync(nd, t->dbox, sizeof(T) * 2 * D * t->nnodes, hipMemcpyDeviceToDevice, s));
             CKD_HIP(hipMemcpyAsync(nr, t->range, sizeof(Range) * t->nnodes, hipMemcpyDeviceToDevice, s));
-            CKD_HIP(hipStreamSynchronize(s));
-            (void)hipFree(t->nodes);
-            (void)hipFree(t->bbox);
-            (void)hipFree(t->dbox);
-            (void)hipFree(t->range);
+            for (void *x : {t->nodes, t->bbox, t->dbox, (void *)t->range}) dev_free(x, s);
         }
         t->nodes = nn;
         t->bbox = nb;
@@ -756,8 +792,8 @@ int build_impl(ckd_tree *t, const T *in, hipStream_t s) {
         // per-level work arrays, sized for the largest possible level
         const int64_t nact_max = (n / (leaf + 1) + 1 > nsets ? n / (leaf + 1) + 1 : nsets);
         const int64_t nchunk_max = n / CH + nact_max + 1;
-        DevBuf act, act2, nch, chunk_base, chunks, cstat, dec, cleft, cleft_off, child_active, child_pos, scratch,
-            dev_nact;
+        DevBuf act(s), act2(s), nch(s), chunk_base(s), chunks(s), cstat(s), dec(s), cleft(s), cleft_off(s),
+            child_active(s), child_pos(s), scratch(s);
         if ((rc = act.alloc(sizeof(int32_t) * nact_max)) || (rc = act2.alloc(sizeof(int32_t) * nact_max)) ||
             (rc = nch.alloc(sizeof(int64_t) * nact_max)) ||
             (rc = chunk_base.alloc(sizeof(int64_t) * (nact_max + 1))) ||
@@ -772,7 +808,7 @@ int build_impl(ckd_tree *t, const T *in, hipStream_t s) {
         IdxT *Ic = I.as<IdxT>(), *In = I2.as<IdxT>();
         int32_t *actc = act.as<int32_t>(), *actn = act2.as<int32_t>();
         int64_t *h_nact = nullptr;
-        CKD_HIP(hipHostMalloc((void **)&h_nact, sizeof(int64_t), hipHostMallocDefault));
+        if ((rc = pinned_slot(&h_nact))) return rc;
         // roots: one per point set; the active ones start the level loop
         k_init_roots<T, D><<<(unsigned)ceil_div(nsets, 256), 256, 0, s>>>(
             (NodeRec<T> *)t->nodes, (T *)t->bbox, t->range, t->set_off, nsets, tvL, t->periodic, leaf,
@@ -780,10 +816,8 @@ int build_impl(ckd_tree *t, const T *in, hipStream_t s) {
         k_root_leaves<T, D, IdxT><<<(unsigned)nsets, BT, 0, s>>>(Pc, Ic, n, t->set_off, child_active.as<int64_t>(),
                                                                  (T *)t->bbox, !t->periodic, (T *)t->pts,
                                                                  (IdxT *)t->idx);
-        if ((rc = scan_i64(child_active.as<int64_t>(), child_pos.as<int64_t>(), nsets, true, scratch.p, s))) {
-            (void)hipHostFree(h_nact);
+        if ((rc = scan_i64(child_active.as<int64_t>(), child_pos.as<int64_t>(), nsets, true, scratch.p, s)))
             return rc;
-        }
         k_compact_active<<<(unsigned)ceil_div(nsets, 256), 256, 0, s>>>(
             child_active.as<int64_t>(), child_pos.as<int64_t>(), nsets, 0, actc);
         CKD_HIP(hipGetLastError());
@@ -801,15 +835,11 @@ int build_impl(ckd_tree *t, const T *in, hipStream_t s) {
                 int64_t nc = cap * 2;
                 while (nc < (int64_t)next_id + 2 * nact) nc *= 2;
                 if (nc > INT32_MAX) {
-                    (void)hipHostFree(h_nact);
                     set_error("tree needs more than 2^31 nodes");
                     return CKD_ERR_LIMIT;
                 }
                 t->nnodes = next_id;
-                if ((rc = alloc_nodes(nc))) {
-                    (void)hipHostFree(h_nact);
-                    return rc;
-                }
+                if ((rc = alloc_nodes(nc))) return rc;
             }
             const unsigned ga = (unsigned)ceil_div(nact, 256);
             const int64_t nub = n / CH + nact + 1;  // upper bound of chunks this level
@@ -820,9 +850,14 @@ int build_impl(ckd_tree *t, const T *in, hipStream_t s) {
             const int64_t *d_ntot = d_cb + nact;  // total chunks (device)
             const unsigned gch = (unsigned)ceil_div(nub, CPB);  // one wave per chunk slot
             k_chunk_stats<T, D><<<gch, BT, 0, s>>>(chunks.as<Chunk>(), d_ntot, Pc, n, cstat.as<Ext<T>>());
-            k_node_decide<T, D><<<(unsigned)ceil_div(nact, CPB), BT, 0, s>>>(actc, nact, d_cb, d_nch, cstat.as<Ext<T>>(),
-                                                              (NodeRec<T> *)t->nodes, (T *)t->bbox, (T *)t->dbox,
-                                                              dec.as<Decision<T>>(), level == 0 && !t->periodic);
+            if (nact * NODE_THREAD_MIN_CH >= nub)  // at most a few chunks per node: one thread per node
+                k_node_decide_t<T, D><<<(unsigned)ceil_div(nact, BT), BT, 0, s>>>(
+                    actc, nact, d_cb, d_nch, cstat.as<Ext<T>>(), (NodeRec<T> *)t->nodes, (T *)t->bbox, (T *)t->dbox,
+                    dec.as<Decision<T>>(), level == 0 && !t->periodic);
+            else
+                k_node_decide<T, D><<<(unsigned)ceil_div(nact, CPB), BT, 0, s>>>(
+                    actc, nact, d_cb, d_nch, cstat.as<Ext<T>>(), (NodeRec<T> *)t->nodes, (T *)t->bbox, (T *)t->dbox,
+                    dec.as<Decision<T>>(), level == 0 && !t->periodic);
             k_chunk_countleft<T><<<gch, BT, 0, s>>>(chunks.as<Chunk>(), d_ntot, nub, Pc, n,
                                                                dec.as<Decision<T>>(), cleft.as<int64_t>());
             if ((rc = scan_i64(cleft.as<int64_t>(), cleft_off.as<int64_t>(), nub, true, scratch.p, s))) break;
@@ -851,7 +886,6 @@ int build_impl(ckd_tree *t, const T *in, hipStream_t s) {
             std::swap(actc, actn);
             level++;
         }
-        (void)hipHostFree(h_nact);
         if (rc) return rc;
         t->nnodes = next_id;
         t->nleaves = nleaves;
@@ -867,8 +901,6 @@ int build_impl(ckd_tree *t, const T *in, hipStream_t s) {
     float ms = 0;
     CKD_HIP(hipEventElapsedTime(&ms, ev0, ev1));
     t->build_ms = ms;
-    (void)hipEventDestroy(ev0);
-    (void)hipEventDestroy(ev1);
     return CKD_OK;
 }
 

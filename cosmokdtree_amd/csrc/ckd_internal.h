@@ -89,6 +89,17 @@ int hip_fail(hipError_t e, const char *what, const char *file, int line);
         if (e_ != hipSuccess) return hip_fail(e_, #call, __FILE__, __LINE__); \
     } while (0)
 
+// ------------------------------------------------------ device memory
+// Device memory from the library's caching allocator on the current device
+// (freed blocks stay mapped for the next allocation; ckd_release_memory
+// returns them to the driver).  dev_alloc's block is ready for work
+// enqueued on `s` after the call.  dev_free hands a block back once the
+// work enqueued on `s` before the free has run; dev_free_idle hands back a
+// block that no pending work uses (e.g. after a device synchronisation).
+int dev_alloc(void **p, size_t bytes, hipStream_t s);
+void dev_free(void *p, hipStream_t s);
+void dev_free_idle(void *p);
+
 // ---------------------------------------------------- device arithmetic
 // |x|, min, max as single instructions (source modifiers, v_min/v_max);
 // the operands are never NaN, and each returns one of its inputs exactly.

@@ -154,8 +154,8 @@ int ckd_tree_load(const char *path, void *stream, ckd_tree **out) {
             rc = CKD_ERR_ARG;
             break;
         }
-        hipError_t e = hipMalloc(a.ptr, a.bytes);
-        if (e == hipSuccess) e = hipMemcpyAsync(*a.ptr, host.data(), a.bytes, hipMemcpyHostToDevice, (hipStream_t)stream);
+        if ((rc = dev_alloc(a.ptr, a.bytes, (hipStream_t)stream))) break;
+        hipError_t e = hipMemcpyAsync(*a.ptr, host.data(), a.bytes, hipMemcpyHostToDevice, (hipStream_t)stream);
         if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
         if (e != hipSuccess) {
             rc = hip_fail(e, "loading a tree", __FILE__, __LINE__);

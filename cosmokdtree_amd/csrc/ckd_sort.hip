@@ -141,12 +141,16 @@ int segmented_sort(T *keys, int64_t *vals, const int64_t *off, int64_t nrows, in
     int64_t maxlen = width;
     if (off) {
         unsigned long long *dmx = nullptr, hmx = 0;
-        CKD_HIP(hipMalloc(&dmx, sizeof(unsigned long long)));
-        CKD_HIP(hipMemsetAsync(dmx, 0, sizeof(unsigned long long), s));
-        k_max_row<T><<<(unsigned)ceil_div(nrows, 256), 256, 0, s>>>(off, nrows, dmx);
-        CKD_HIP(hipMemcpyAsync(&hmx, dmx, sizeof(hmx), hipMemcpyDeviceToHost, s));
-        CKD_HIP(hipStreamSynchronize(s));
-        (void)hipFree(dmx);
+        int rc = dev_alloc((void **)&dmx, sizeof(unsigned long long), s);
+        if (rc) return rc;
+        hipError_t e = hipMemsetAsync(dmx, 0, sizeof(unsigned long long), s);
+        if (e == hipSuccess) {
+            k_max_row<T><<<(unsigned)ceil_div(nrows, 256), 256, 0, s>>>(off, nrows, dmx);
+            e = hipMemcpyAsync(&hmx, dmx, sizeof(hmx), hipMemcpyDeviceToHost, s);
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        dev_free(dmx, s);
+        if (e != hipSuccess) return hip_fail(e, "longest row", __FILE__, __LINE__);
         maxlen = (int64_t)hmx;
     }
     if (maxlen <= LDS_MAX) return CKD_OK;
@@ -155,23 +159,28 @@ int segmented_sort(T *keys, int64_t *vals, const int64_t *off, int64_t nrows, in
     if (off) CKD_HIP(hipMemcpy(&total, off + nrows, sizeof(int64_t), hipMemcpyDeviceToHost));
     T *tk = nullptr;
     int64_t *tv = nullptr;
-    CKD_HIP(hipMalloc(&tk, sizeof(T) * total));
-    CKD_HIP(hipMalloc(&tv, sizeof(int64_t) * total));
+    int rc = dev_alloc((void **)&tk, sizeof(T) * total, s);
+    if (!rc) rc = dev_alloc((void **)&tv, sizeof(int64_t) * total, s);
+    if (rc) {
+        dev_free(tk, s);
+        return rc;
+    }
     T *sk = keys, *dk = tk;
     int64_t *sv = vals, *dv = tv;
-    for (int64_t run = LDS_MAX; run < maxlen; run *= 2) {
+    hipError_t e = hipSuccess;
+    for (int64_t run = LDS_MAX; run < maxlen && e == hipSuccess; run *= 2) {
         k_merge_pass<T><<<(unsigned)nrows, ST, 0, s>>>(sk, sv, dk, dv, off, width, run);
-        CKD_HIP(hipGetLastError());
+        e = hipGetLastError();
         std::swap(sk, dk);
         std::swap(sv, dv);
     }
-    if (sk != keys) {  // rows longer than LDS_MAX ended in the temporary buffer
-        CKD_HIP(hipMemcpyAsync(keys, sk, sizeof(T) * total, hipMemcpyDeviceToDevice, s));
-        CKD_HIP(hipMemcpyAsync(vals, sv, sizeof(int64_t) * total, hipMemcpyDeviceToDevice, s));
+    if (e == hipSuccess && sk != keys) {  // rows longer than LDS_MAX ended in the temporary buffer
+        e = hipMemcpyAsync(keys, sk, sizeof(T) * total, hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(vals, sv, sizeof(int64_t) * total, hipMemcpyDeviceToDevice, s);
     }
-    CKD_HIP(hipStreamSynchronize(s));
-    (void)hipFree(tk);
-    (void)hipFree(tv);
+    dev_free(tk, s);  // after the merge passes and copies enqueued above
+    dev_free(tv, s);
+    if (e != hipSuccess) return hip_fail(e, "merge passes", __FILE__, __LINE__);
     return CKD_OK;
 }
 

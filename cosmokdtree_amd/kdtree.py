@@ -269,6 +269,17 @@ def load_tree(path: str) -> KDTree:
     return tree
 
 
+def release_memory() -> None:
+    """Return the library's cached but unused device memory to the driver.
+
+    Builds and queries take their device memory from a per-device cache
+    that keeps freed blocks for the next call; live trees keep theirs.  Call
+    this before a phase that needs the HBM elsewhere (torch's caching
+    allocator does not see the library's cache).  Waits for all work on the
+    devices used."""
+    check(lib.ckd_release_memory(), "release memory")
+
+
 # --------------------------------------------------------------- drop-in API
 
 def build_kdtree(points, leaf=None, boxsize=None):

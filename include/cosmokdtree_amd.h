@@ -325,6 +325,20 @@ int ckd_mlp_grad(const float *params, const uint16_t *X, const float *target, in
    `step` >= 1) on params with moment buffers m, v [dev, nparam float]. */
 int ckd_mlp_adam(float *params, const float *grad, float *m, float *v, int64_t step, float lr, void *stream);
 
+/*
+ * Device memory: every device array of a tree, and the scratch of builds
+ * and queries, comes from a caching allocator (one free list per device)
+ * that keeps freed blocks mapped, so that repeated builds and queries take
+ * their memory without hipMalloc/hipFree (page mapping, device-wide
+ * synchronisation).  This call waits for the work on every device the
+ * library has used and returns the cached blocks to the driver (memory of
+ * live trees stays).  An allocation that finds the device full does the
+ * same for its device before it fails.  No reference counterpart (the
+ * reference allocates host arrays with Fortran ALLOCATE,
+ * cosmokdtree.f90:151-153,206-211).
+ */
+int ckd_release_memory(void);
+
 /* Text of the last error on this thread ("" if none). */
 const char *ckd_last_error(void);
 
