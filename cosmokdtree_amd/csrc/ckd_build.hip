@@ -33,6 +33,7 @@ constexpr int CH = 2048;          // points per chunk
 constexpr int BT = 256;           // threads per workgroup
 constexpr int CPB = BT / WAVE;    // chunks (or nodes) per workgroup, one per wave
 constexpr int64_t NODE_THREAD_MIN_CH = 4;  // decide with one thread per node when chunks <= 4 x nodes
+constexpr int RU = 4;             // rounds of 64 points whose loads a chunk wave issues together
 
 struct Chunk {
     int64_t begin, end;  // positions in the working buffers
@@ -164,12 +165,24 @@ __global__ __launch_bounds__(BT) void k_chunk_stats(const Chunk *chunks, const i
     Ext<T> e[D];
 #pragma unroll
     for (int d = 0; d < D; d++) e[d] = ext_empty<T>();
-    for (int64_t p = ch.begin + lane; p < ch.end; p += WAVE) {
+    for (int64_t base = ch.begin; base < ch.end; base += RU * WAVE) {
+        T x[RU][D];  // RU rounds' loads in flight together
 #pragma unroll
-        for (int d = 0; d < D; d++) {
-            T x = P[(int64_t)d * n + p];
-            if (x < e[d].mn) { e[d].mn = x; e[d].mnpos = p; }
-            if (x > e[d].mx) { e[d].mx = x; e[d].mxpos = p; }
+        for (int u = 0; u < RU; u++) {
+            const int64_t p = base + u * WAVE + lane;
+            const int64_t q = p < ch.end ? p : base;
+#pragma unroll
+            for (int d = 0; d < D; d++) x[u][d] = P[(int64_t)d * n + q];
+        }
+#pragma unroll
+        for (int u = 0; u < RU; u++) {  // rounds in order: first positions stay first
+            const int64_t p = base + u * WAVE + lane;
+            if (p < ch.end)
+#pragma unroll
+                for (int d = 0; d < D; d++) {
+                    if (x[u][d] < e[d].mn) { e[d].mn = x[u][d]; e[d].mnpos = p; }
+                    if (x[u][d] > e[d].mx) { e[d].mx = x[u][d]; e[d].mxpos = p; }
+                }
         }
     }
     wave_reduce_ext<T, D>(e);
@@ -286,10 +299,18 @@ __global__ __launch_bounds__(BT) void k_chunk_countleft(const Chunk *chunks, con
     const Decision<T> dc = dec[ch.act];
     const T *col = P + (int64_t)dc.axis * n;
     int64_t c = 0;
-    for (int64_t base = ch.begin; base < ch.end; base += WAVE) {
-        const int64_t p = base + lane;
-        const bool f = p < ch.end && goes_left(dc, col[p < ch.end ? p : base], p);
-        c += __popcll(__ballot(f));
+    for (int64_t base = ch.begin; base < ch.end; base += RU * WAVE) {
+        T x[RU];
+#pragma unroll
+        for (int u = 0; u < RU; u++) {
+            const int64_t p = base + u * WAVE + lane;
+            x[u] = col[p < ch.end ? p : base];
+        }
+#pragma unroll
+        for (int u = 0; u < RU; u++) {
+            const int64_t p = base + u * WAVE + lane;
+            c += __popcll(__ballot(p < ch.end && goes_left(dc, x[u], p)));
+        }
     }
     if (lane == 0) cleft[ci] = c;
 }
@@ -316,25 +337,40 @@ __global__ __launch_bounds__(BT) void k_chunk_scatter(const Chunk *chunks, const
     const int64_t left_base = nstart + (cleft_off[ci] - cleft_off[first]);
     const int64_t n_left = cleft_off[first + nch[ch.act]] - cleft_off[first];
     const int64_t right_base = nstart + n_left + (ch.begin - nstart) - (cleft_off[ci] - cleft_off[first]);
-    const T *col = P + (int64_t)dc.axis * n;
     int64_t lacc = 0, racc = 0;
-    for (int64_t base = ch.begin; base < ch.end; base += WAVE) {
-        const int64_t p = base + lane;
-        const bool in = p < ch.end;
-        const int64_t q = in ? p : base;  // a valid row for the idle lanes' loads
-        const bool f = in && goes_left(dc, col[q], p);
-        const uint64_t m = __ballot(f);
-        const int rank = __popcll(m & below);
-        const int nl = __popcll(m);
-        const int cnt = ch.end - base < WAVE ? (int)(ch.end - base) : WAVE;
-        if (in) {
-            const int64_t dst = f ? left_base + lacc + rank : right_base + racc + (lane - rank);
+    for (int64_t base = ch.begin; base < ch.end; base += RU * WAVE) {
+        T x[RU][D];  // RU rounds' loads in flight together
+        IdxT iv[RU];
 #pragma unroll
-            for (int d = 0; d < D; d++) P2[(int64_t)d * n + dst] = P[(int64_t)d * n + p];
-            I2[dst] = I[p];
+        for (int u = 0; u < RU; u++) {
+            const int64_t p = base + u * WAVE + lane;
+            const int64_t q = p < ch.end ? p : base;  // a valid row for the idle lanes' loads
+#pragma unroll
+            for (int d = 0; d < D; d++) x[u][d] = P[(int64_t)d * n + q];
+            iv[u] = I[q];
         }
-        lacc += nl;
-        racc += cnt - nl;
+#pragma unroll
+        for (int u = 0; u < RU; u++) {
+            const int64_t rb = base + u * WAVE, p = rb + lane;
+            const bool in = p < ch.end;
+            T xa = x[u][0];
+#pragma unroll
+            for (int d = 1; d < D; d++)
+                if (d == dc.axis) xa = x[u][d];
+            const bool f = in && goes_left(dc, xa, p);
+            const uint64_t m = __ballot(f);
+            const int rank = __popcll(m & below);
+            const int nl = __popcll(m);
+            const int cnt = ch.end - rb >= WAVE ? WAVE : (ch.end > rb ? (int)(ch.end - rb) : 0);
+            if (in) {
+                const int64_t dst = f ? left_base + lacc + rank : right_base + racc + (lane - rank);
+#pragma unroll
+                for (int d = 0; d < D; d++) P2[(int64_t)d * n + dst] = x[u][d];
+                I2[dst] = iv[u];
+            }
+            lacc += nl;
+            racc += cnt - nl;
+        }
     }
 }
 
