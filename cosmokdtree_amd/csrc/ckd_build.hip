@@ -34,6 +34,7 @@ constexpr int BT = 256;           // threads per workgroup
 constexpr int CPB = BT / WAVE;    // chunks (or nodes) per workgroup, one per wave
 constexpr int64_t NODE_THREAD_MIN_CH = 4;  // decide with one thread per node when chunks <= 4 x nodes
 constexpr int RU = 4;             // rounds of 64 points whose loads a chunk wave issues together
+constexpr int STATS_G = 16;       // lanes per chunk of the extremes pass on levels of small nodes
 
 struct Chunk {
     int64_t begin, end;  // positions in the working buffers
@@ -152,6 +153,23 @@ __global__ void k_chunk_fill(const int32_t *act, const Range *range, const int64
     }
 }
 
+// One thread per active node (levels whose nodes have one or a few chunks).
+__global__ void k_chunk_fill_t(const int32_t *act, int64_t nact, const Range *range, const int64_t *chunk_base,
+                               Chunk *chunks) {
+    const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= nact) return;
+    const Range r = range[act[a]];
+    const int64_t nc = ceil_div(r.count, CH), base = chunk_base[a];
+    for (int64_t c = 0; c < nc; c++) {
+        Chunk ch;
+        ch.begin = r.start + c * CH;
+        ch.end = r.start + (c + 1) * CH < r.start + r.count ? r.start + (c + 1) * CH : r.start + r.count;
+        ch.act = (int32_t)a;
+        ch.pad = 0;
+        chunks[base + c] = ch;
+    }
+}
+
 // One wave per chunk: lanes take points p = begin + lane, + 64, ...
 // (coalesced column reads); strict comparisons keep each lane's first
 // position, ext_merge the smallest position on ties.
@@ -187,6 +205,55 @@ __global__ __launch_bounds__(BT) void k_chunk_stats(const Chunk *chunks, const i
     }
     wave_reduce_ext<T, D>(e);
     if (lane == 0)
+#pragma unroll
+        for (int d = 0; d < D; d++) cstat[ci * D + d] = e[d];
+}
+
+// Same with G lanes per chunk (WAVE / G chunks per wave), for levels of
+// small nodes where most chunks hold a few dozen points: shorter
+// reductions and fewer waves.
+template <typename T, int D, int G>
+__global__ __launch_bounds__(BT) void k_chunk_stats_g(const Chunk *chunks, const int64_t *nchunks, const T *P,
+                                                      int64_t n, Ext<T> *cstat) {
+    const int64_t ci = ((int64_t)blockIdx.x * BT + threadIdx.x) / G;
+    if (ci >= *nchunks) return;  // whole group
+    const int gl = threadIdx.x & (G - 1);
+    const Chunk ch = chunks[ci];
+    Ext<T> e[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) e[d] = ext_empty<T>();
+    for (int64_t base = ch.begin; base < ch.end; base += RU * G) {
+        T x[RU][D];
+#pragma unroll
+        for (int u = 0; u < RU; u++) {
+            const int64_t p = base + u * G + gl;
+            const int64_t q = p < ch.end ? p : base;
+#pragma unroll
+            for (int d = 0; d < D; d++) x[u][d] = P[(int64_t)d * n + q];
+        }
+#pragma unroll
+        for (int u = 0; u < RU; u++) {
+            const int64_t p = base + u * G + gl;
+            if (p < ch.end)
+#pragma unroll
+                for (int d = 0; d < D; d++) {
+                    if (x[u][d] < e[d].mn) { e[d].mn = x[u][d]; e[d].mnpos = p; }
+                    if (x[u][d] > e[d].mx) { e[d].mx = x[u][d]; e[d].mxpos = p; }
+                }
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < D; d++)
+#pragma unroll
+        for (int m = G / 2; m >= 1; m >>= 1) {
+            Ext<T> o;
+            o.mn = __shfl_xor(e[d].mn, m, G);
+            o.mx = __shfl_xor(e[d].mx, m, G);
+            o.mnpos = __shfl_xor(e[d].mnpos, m, G);
+            o.mxpos = __shfl_xor(e[d].mxpos, m, G);
+            ext_merge(e[d], o);
+        }
+    if (gl == 0)
 #pragma unroll
         for (int d = 0; d < D; d++) cstat[ci * D + d] = e[d];
 }
@@ -882,11 +949,20 @@ int build_impl(ckd_tree *t, const T *in, hipStream_t s) {
             int64_t *d_nch = nch.as<int64_t>(), *d_cb = chunk_base.as<int64_t>();
             k_chunk_count<<<ga, 256, 0, s>>>(actc, nact, t->range, d_nch);
             if ((rc = scan_i64(d_nch, d_cb, nact, true, scratch.p, s))) break;
-            k_chunk_fill<<<(unsigned)nact, 64, 0, s>>>(actc, t->range, d_cb, chunks.as<Chunk>());
+            // levels of small nodes: at most a few chunks per node
+            const bool small_nodes = nact * NODE_THREAD_MIN_CH >= nub;
+            if (small_nodes)
+                k_chunk_fill_t<<<ga, 256, 0, s>>>(actc, nact, t->range, d_cb, chunks.as<Chunk>());
+            else
+                k_chunk_fill<<<(unsigned)nact, 64, 0, s>>>(actc, t->range, d_cb, chunks.as<Chunk>());
             const int64_t *d_ntot = d_cb + nact;  // total chunks (device)
             const unsigned gch = (unsigned)ceil_div(nub, CPB);  // one wave per chunk slot
-            k_chunk_stats<T, D><<<gch, BT, 0, s>>>(chunks.as<Chunk>(), d_ntot, Pc, n, cstat.as<Ext<T>>());
-            if (nact * NODE_THREAD_MIN_CH >= nub)  // at most a few chunks per node: one thread per node
+            if (small_nodes)  // 16 lanes per chunk
+                k_chunk_stats_g<T, D, STATS_G><<<(unsigned)ceil_div(nub * STATS_G, BT), BT, 0, s>>>(
+                    chunks.as<Chunk>(), d_ntot, Pc, n, cstat.as<Ext<T>>());
+            else
+                k_chunk_stats<T, D><<<gch, BT, 0, s>>>(chunks.as<Chunk>(), d_ntot, Pc, n, cstat.as<Ext<T>>());
+            if (small_nodes)  // one thread per node
                 k_node_decide_t<T, D><<<(unsigned)ceil_div(nact, BT), BT, 0, s>>>(
                     actc, nact, d_cb, d_nch, cstat.as<Ext<T>>(), (NodeRec<T> *)t->nodes, (T *)t->bbox, (T *)t->dbox,
                     dec.as<Decision<T>>(), level == 0 && !t->periodic);
