@@ -324,9 +324,21 @@ __global__ __launch_bounds__(BT) void k_node_decide(const int32_t *act, int64_t 
 #pragma unroll
     for (int d = 0; d < D; d++) e[d] = ext_empty<T>();
     const int64_t c0 = chunk_base[a], c1 = c0 + nch[a];
-    for (int64_t c = c0 + lane; c < c1; c += WAVE)
+    for (int64_t base = c0; base < c1; base += RU * WAVE) {
+        Ext<T> x[RU][D];  // RU rounds' loads in flight together
 #pragma unroll
-        for (int d = 0; d < D; d++) ext_merge(e[d], cstat[c * D + d]);
+        for (int u = 0; u < RU; u++) {
+            const int64_t c = base + u * WAVE + lane;
+            const int64_t q = c < c1 ? c : base;
+#pragma unroll
+            for (int d = 0; d < D; d++) x[u][d] = cstat[q * D + d];
+        }
+#pragma unroll
+        for (int u = 0; u < RU; u++)
+            if (base + u * WAVE + lane < c1)
+#pragma unroll
+                for (int d = 0; d < D; d++) ext_merge(e[d], x[u][d]);
+    }
     wave_reduce_ext<T, D>(e);
     if (lane == 0) decide_node<T, D>(a, act[a], e, nodes, bbox, dbox, dec, root_from_data);
 }
