@@ -253,12 +253,15 @@ int run_t(ckd_batch *b, const T *queries, T *kdist, int64_t *kidx, int64_t *ball
     if (b->has_ball) {
         int rc = scan_i64(b->counts, ball_off, b->nq, true, b->scan_tmp, s);
         if (rc) return rc;
-        if (b->n_b > 0)
+        if (b->n_b > 0) {
             k_batch<T, D, PER, NT, SD, 1><<<b->grid, NT, 0, s>>>(tv, t->set_off, queries, b->items_b, b->n_b,
                                                                  b->counters + 1, b->kset, b->rset, b->qoff,
                                                                  b->knn_off, b->sorted, kdist, kidx, b->counts,
                                                                  ball_off, bdist, bidx, b->capacity, b->counters + 2,
                                                                  order, b->leaf_of, 0, 1);
+            // the fill walks wrote d^2; rows past the capacity were not written
+            k_sqrt_rows<T><<<SQRT_BLOCKS, 256, 0, s>>>(bdist, ball_off + b->nq, b->capacity);
+        }
         CKD_HIP(hipGetLastError());
     }
     return CKD_OK;

@@ -723,6 +723,7 @@ int ball_launch(const ckd_tree *t, const T *q, int64_t nq, T r2, int64_t *counts
                     else k_ball_s<T, D, false, false, 64><<<g, QT, 0, s>>>(tv, root, rows, q, nq, order, r2, counts, off, dist, idx);
                 }
             }
+            if (fill) k_sqrt_rows<T><<<SQRT_BLOCKS, 256, 0, s>>>(dist, off + nq, INT64_MAX);
             CKD_HIP(hipGetLastError());
             return CKD_OK;
         }
@@ -734,6 +735,7 @@ int ball_launch(const ckd_tree *t, const T *q, int64_t nq, T r2, int64_t *counts
         if (fill) k_ball<T, D, false, true><<<g, QT, 0, s>>>(tv, q, nq, r2, counts, off, dist, idx);
         else k_ball<T, D, false, false><<<g, QT, 0, s>>>(tv, q, nq, r2, counts, off, dist, idx);
     }
+    if (fill) k_sqrt_rows<T><<<SQRT_BLOCKS, 256, 0, s>>>(dist, off + nq, INT64_MAX);
     CKD_HIP(hipGetLastError());
     return CKD_OK;
 }

@@ -790,14 +790,17 @@ __device__ __forceinline__ bool knn_stack(const TreeView<T> &tv, int2 root, cons
 // :1332-1366), leaves test d^2 <= r^2 inclusively (:1126).  The box bounds
 // use the same rounded per-axis differences as the point distance, so the
 // whole-range rule includes exactly the points the per-point test would.  Returns the count; with
-// FILL writes (sqrt d^2, index - idx_base) from dout/iout on.
+// FILL writes (d^2, index - idx_base) from dout/iout on: the square roots
+// are taken afterwards over the whole output by k_sqrt_rows, so the walk
+// does not run the correctly rounded square root for every row a lane of
+// the wave accepts.
 template <typename T, int D, bool PER, bool FILL>
 __device__ __forceinline__ int64_t ball_one(const TreeView<T> &tv, const T (&q)[D], int32_t root, T r2, T *dout,
                                             int64_t *iout, int64_t idx_base) {
     int64_t cnt = 0, w = 0;
     auto emit = [&](int64_t p, T d) {
         if (FILL) {
-            dout[w] = sqrt_rn<T>(d);
+            dout[w] = d;
             iout[w] = load_idx(tv.idx, tv.idx64, p) - idx_base;
             w++;
         }
@@ -888,14 +891,15 @@ __device__ __forceinline__ void drain_emits(const TreeView<T> &tv, const T (&q)[
             const int32_t p = E.start + j;
             T x[D];
             load_pt<T, D>(tv.pts, p, x);
-            dout[E.out + j] = sqrt_rn<T>(dist2<T, D, PER>(x, qo, tv.L));
+            dout[E.out + j] = dist2<T, D, PER>(x, qo, tv.L);  // squared, see ball_one
             iout[E.out + j] = load_idx(tv.idx, tv.idx64, p) - idx_base;
         }
     }
     qn = 0;
 }
 
-// Returns this lane's count; with FILL writes its rows from dout/iout + w0.
+// Returns this lane's count; with FILL writes its rows (d^2, index) from
+// dout/iout + w0.
 // rootbox/rootrows: the tight box and rows of the subtree at `root`.
 template <typename T, int D, bool PER, bool FILL, int SD>
 __device__ __forceinline__ int64_t ball_stack(const TreeView<T> &tv, int2 root, const T *rootbox, int2 rootrows,
@@ -944,7 +948,7 @@ __device__ __forceinline__ int64_t ball_stack(const TreeView<T> &tv, int2 root, 
                 const T d = dist2<T, D, PER>(x, q, tv.L);
                 if (run && d <= r2) {
                     if (FILL) {
-                        dout[w] = sqrt_rn<T>(d);
+                        dout[w] = d;  // squared, see ball_one
                         iout[w] = load_idx(tv.idx, tv.idx64, p) - idx_base;
                         w++;
                     }
@@ -1059,5 +1063,16 @@ __device__ __forceinline__ int64_t box_one(const TreeView<T> &tv, const T (&bx)[
     }
     return cnt;
 }
+
+// Square roots of the ball rows, in place: rows [0, min(*total, cap)) of
+// dout hold d^2 from the fill walks (ball_one, ball_stack).  A grid-stride
+// loop, so the count stays on the device.
+template <typename T>
+__global__ void k_sqrt_rows(T *dout, const int64_t *total, int64_t cap) {
+    const int64_t n = *total < cap ? *total : cap;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dout[i] = sqrt_rn<T>(dout[i]);
+}
+constexpr int SQRT_BLOCKS = 2048;  // k_sqrt_rows grid (256 threads each)
 
 }  // namespace ckd
