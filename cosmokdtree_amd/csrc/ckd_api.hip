@@ -67,8 +67,8 @@ static void release(ckd_tree *t) {
     (void)hipGetDevice(&cur);
     if (cur != t->device) (void)hipSetDevice(t->device);
     (void)hipDeviceSynchronize();
-    void *arrays[] = {t->set_off, t->ws[0], t->ws[1], t->ws[2], t->nodes, t->bbox, t->dbox, t->range,
-                      t->pts,     t->idx,   t->wide,  t->wroot, t->wrange};
+    void *arrays[] = {t->set_off, t->ws[0], t->ws[1], t->ws[2], t->nodes,  t->bbox,  t->dbox,
+                      t->range,   t->pts,   t->idx,   t->wide,  t->wroot, t->wrange, t->wparent};
     static_assert(ckd_tree::NWS == 3, "workspace slots listed above");
     for (void *a : arrays) dev_free_idle(a);
     if (cur != t->device) (void)hipSetDevice(cur);
@@ -79,6 +79,7 @@ static void release(ckd_tree *t) {
     }
     t->nodes = t->bbox = t->dbox = t->pts = t->idx = t->wide = nullptr;
     t->wroot = t->wrange = nullptr;
+    t->wparent = nullptr;
     t->nwide = 0;
     t->range = nullptr;
 }

@@ -178,8 +178,12 @@ int ckd_batch_status(const ckd_batch *b, const int64_t *ball_off, int64_t *ball_
     if (b->has_ball && ball_off)
         CKD_HIP(hipMemcpyAsync(&tot, ball_off + b->nq, sizeof(int64_t), hipMemcpyDeviceToHost, s));
     CKD_HIP(hipStreamSynchronize(s));
-    if (overflow) *overflow = flag ? 1 : 0;
+    if (overflow) *overflow = (flag & 1) ? 1 : 0;
     if (ball_total) *ball_total = tot;
+    if (flag & 2) {  // a ball walk ran past its step bound: the tree's parent links are inconsistent
+        set_error("ball walk exceeded its step bound (corrupt wide records)");
+        return CKD_ERR_HIP;
+    }
     return CKD_OK;
 }
 

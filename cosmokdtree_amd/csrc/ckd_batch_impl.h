@@ -148,7 +148,7 @@ __global__ __launch_bounds__(NT, 3) void k_batch(TreeView<T> tv, const int64_t *
                                               int64_t *ball_counts, const int64_t *ball_off, T *bdist,
                                               int64_t *bidx, int64_t capacity, int64_t *overflow,
                                               const int32_t *order, const int32_t *leaf_of, int use_stack,
-                                              int Grt) {
+                                              int Grt, int64_t step_limit) {
     const int G = GC ? GC : Grt;
     __shared__ int64_t s_item;
     constexpr int CBMAX = sizeof(T) == 4 ? 16 : 8;
@@ -203,7 +203,21 @@ __global__ __launch_bounds__(NT, 3) void k_batch(TreeView<T> tv, const int64_t *
         } else {
             const T r = (T)rset[w.set];
             const T r2 = r * r;
-            if (w.kind == 1) {
+            if (tv.wparent) {  // over the wide records (present when rows fit int32)
+                const int2 rp = tv.wroot[w.set];
+                const T *rb = tv.dbox + (int64_t)w.set * 2 * D;
+                bool bad = false;
+                if (w.kind == 1) {
+                    ball_counts[qg] = ball_wide_one<T, D, PER, false>(tv, q, rp, rb, r2, nullptr, nullptr, base,
+                                                                      step_limit, bad);
+                } else {
+                    const int64_t o = ball_off[qg];
+                    if (ball_off[qg + 1] <= capacity)
+                        ball_wide_one<T, D, PER, true>(tv, q, rp, rb, r2, bdist + o, bidx + o, base, step_limit, bad);
+                    else atomicOr((unsigned long long *)overflow, 1ull);
+                }
+                if (bad) atomicOr((unsigned long long *)overflow, 2ull);
+            } else if (w.kind == 1) {
                 ball_counts[qg] = ball_one<T, D, PER, false>(tv, q, root, r2, nullptr, nullptr, base);
             } else {
                 const int64_t o = ball_off[qg];
@@ -237,17 +251,20 @@ int run_t(ckd_batch *b, const T *queries, T *kdist, int64_t *kidx, int64_t *ball
         order = b->order;
     }
     const int G = WAVE / b->qpw;
+    const int64_t step_limit = 3 * t->nwide + 3;  // bound of the ball walk over the wide records
     if (b->n_a > 0) {
         if (G == 1 || !use_stack)
             k_batch<T, D, PER, NT, SD, 1><<<b->grid, NT, 0, s>>>(tv, t->set_off, queries, b->items_a, b->n_a,
                                                                  b->counters, b->kset, b->rset, b->qoff, b->knn_off,
                                                                  b->sorted, kdist, kidx, b->counts, ball_off, bdist,
-                                                                 bidx, b->capacity, b->counters + 2, order, b->leaf_of, use_stack, 1);
+                                                                 bidx, b->capacity, b->counters + 2, order, b->leaf_of, use_stack, 1,
+                                                                 step_limit);
         else
             k_batch<T, D, PER, NT, SD, 0><<<b->grid, NT, 0, s>>>(tv, t->set_off, queries, b->items_a, b->n_a,
                                                                  b->counters, b->kset, b->rset, b->qoff, b->knn_off,
                                                                  b->sorted, kdist, kidx, b->counts, ball_off, bdist,
-                                                                 bidx, b->capacity, b->counters + 2, order, b->leaf_of, use_stack, G);
+                                                                 bidx, b->capacity, b->counters + 2, order, b->leaf_of, use_stack, G,
+                                                                 step_limit);
     }
     CKD_HIP(hipGetLastError());
     if (b->has_ball) {
@@ -258,7 +275,7 @@ int run_t(ckd_batch *b, const T *queries, T *kdist, int64_t *kidx, int64_t *ball
                                                                  b->counters + 1, b->kset, b->rset, b->qoff,
                                                                  b->knn_off, b->sorted, kdist, kidx, b->counts,
                                                                  ball_off, bdist, bidx, b->capacity, b->counters + 2,
-                                                                 order, b->leaf_of, 0, 1);
+                                                                 order, b->leaf_of, 0, 1, step_limit);
             // the fill walks wrote d^2; rows past the capacity were not written
             k_sqrt_rows<T><<<SQRT_BLOCKS, 256, 0, s>>>(bdist, ball_off + b->nq, b->capacity);
         }

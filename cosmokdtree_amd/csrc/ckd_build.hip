@@ -700,6 +700,21 @@ __global__ void k_fill_wide(const NodeRec<T> *nodes, const T *dbox, const Range 
     *(int4 *)(base + 4 * D * sizeof(T)) = make_int4(p0.x, p0.y, p1.x, p1.y);
 }
 
+__global__ void k_set_i32(int32_t *p, int64_t n, int32_t v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// Parent of each inner child, from the child pairs of the wide records.
+__global__ void k_wide_parents(const char *wide, size_t rec_bytes, size_t pair_off, int64_t nwide,
+                               int32_t *wparent) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nwide) return;
+    const int4 r = *(const int4 *)(wide + (size_t)v * rec_bytes + pair_off);
+    if (r.y < 0) wparent[r.x] = (int32_t)v;
+    if (r.w < 0) wparent[r.z] = (int32_t)v;
+}
+
 template <typename T>
 __global__ void k_root_pairs(const NodeRec<T> *nodes, const Range *range, const int64_t *widx, int64_t nsets,
                              int2 *wroot) {
@@ -822,7 +837,7 @@ int build_wide(ckd_tree *t, hipStream_t s) {
     k_root_pairs<T><<<(unsigned)ceil_div(nsets, 256), 256, 0, s>>>(nodes, t->range, widx.as<int64_t>(), nsets,
                                                                     t->wroot);
     CKD_HIP(hipGetLastError());
-    return CKD_OK;
+    return wide_parents(t, s);
 }
 
 template <typename T, int D, typename IdxT>
@@ -1045,6 +1060,20 @@ int build_dispatch(ckd_tree *t, const void *in, hipStream_t s) {
 }
 
 }  // namespace
+
+int wide_parents(ckd_tree *t, hipStream_t s) {
+    if (!t->wide || t->nwide <= 0) return CKD_OK;
+    dev_free(t->wparent, s);
+    t->wparent = nullptr;
+    int rc = dev_alloc((void **)&t->wparent, sizeof(int32_t) * (size_t)t->nwide, s);
+    if (rc) return rc;
+    const unsigned g = (unsigned)ceil_div(t->nwide, 256);
+    const size_t rec = wide_bytes_rt(t->ndim, t->prec), pair_off = rec - 16;
+    k_set_i32<<<g, 256, 0, s>>>(t->wparent, t->nwide, -1);
+    k_wide_parents<<<g, 256, 0, s>>>((const char *)t->wide, rec, pair_off, t->nwide, t->wparent);
+    CKD_HIP(hipGetLastError());
+    return CKD_OK;
+}
 
 int build_tree(ckd_tree *t, const void *points, hipStream_t s) {
     if (t->prec == 4)

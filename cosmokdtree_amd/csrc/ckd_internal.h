@@ -25,6 +25,9 @@
 //   wroot[nsets]    the (a, b) pair of each set's root, same convention.
 //   wrange[nwide]   (first row, count) of each inner node, by wide id (the
 //                   ball walk takes a whole subtree without descending).
+//   wparent[nwide]  wide id of each inner node's parent, -1 for set roots
+//                   (the per-query ball walk of the batched engine climbs
+//                   by it); derived from the wide records at build and load.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -76,6 +79,7 @@ struct TreeView {
     const char *wide;     // wide records (nullptr when not built)
     const int2 *wroot;    // root pair of each set
     const int2 *wrange;   // rows of each inner node, by wide id
+    const int32_t *wparent;  // parent of each inner node, by wide id (-1 for roots)
     int idx64;
     T L[CKD_MAX_DIM];  // box size per axis (periodic only)
 };
@@ -267,6 +271,7 @@ struct ckd_tree {
     int64_t nwide = 0;
     int2 *wroot = nullptr;   // device [nsets] root pairs
     int2 *wrange = nullptr;  // device [nwide] inner-node rows
+    int32_t *wparent = nullptr;  // device [nwide] parent wide ids, -1 for roots
     int knn_kernel = -1;     // kNN kernel choice, -1 = automatic (ckd_knn_config)
     int knn_stack = 0;       // stack depth of the stack kernel, 0 = automatic
     int knn_qpw = 0;         // queries per wave of the stack kernel (1..64), 0 = 64
@@ -292,6 +297,7 @@ TreeView<T> view(const ckd_tree *t) {
     v.wide = (const char *)t->wide;
     v.wroot = t->wroot;
     v.wrange = t->wrange;
+    v.wparent = t->wparent;
     v.idx64 = t->idx_bits == 64;
     for (int i = 0; i < CKD_MAX_DIM; i++) v.L[i] = (T)t->L[i];
     return v;
@@ -300,4 +306,6 @@ int build_tree(ckd_tree *t, const void *points, hipStream_t s);
 // Device workspace `slot` of at least `bytes` owned by the tree (grown on
 // demand; growing synchronises `s` before releasing the old buffer).
 int workspace(ckd_tree *t, int slot, size_t bytes, void **p, hipStream_t s);
+// wparent from the wide records (after a build or a load of a tree with them).
+int wide_parents(ckd_tree *t, hipStream_t s);
 }  // namespace ckd

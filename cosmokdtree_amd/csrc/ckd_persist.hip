@@ -163,6 +163,13 @@ int ckd_tree_load(const char *path, void *stream, ckd_tree **out) {
         }
     }
     fclose(f);
+    if (rc == CKD_OK && has_wide) {  // parent pointers are derived, not stored
+        rc = wide_parents(t, (hipStream_t)stream);
+        if (rc == CKD_OK) {
+            hipError_t e = hipStreamSynchronize((hipStream_t)stream);
+            if (e != hipSuccess) rc = hip_fail(e, "loading a tree", __FILE__, __LINE__);
+        }
+    }
     if (rc != CKD_OK) {
         ckd_free(t);
         return rc;

@@ -854,6 +854,82 @@ __device__ __forceinline__ int64_t ball_one(const TreeView<T> &tv, const T (&q)[
     return cnt;
 }
 
+// The same ball search (same decisions, same output order) over the wide
+// records, for one query per lane: one record load decides both children
+// of a node, where ball_one loads a child's node, box and range apart.
+// Stack-free like ball_one: the walk climbs by wparent and reads the
+// parent's child pairs to learn which side it came from.  root is the
+// subtree's (a, b) pair (wroot convention), rootbox its tight box.  Every
+// inner node is loaded at most three times (arrival, return from each
+// child), so the walk ends within 3 * nwide + 3 steps; past step_limit it
+// stops and sets `bad` (a corrupt tree) instead of looping.
+template <typename T, int D, bool PER, bool FILL>
+__device__ __forceinline__ int64_t ball_wide_one(const TreeView<T> &tv, const T (&q)[D], int2 root,
+                                                 const T *rootbox, T r2, T *dout, int64_t *iout, int64_t idx_base,
+                                                 int64_t step_limit, bool &bad) {
+    int64_t cnt = 0, w = 0;
+    // rows [s, s + n): every one (a subtree within r) or those within r
+    auto take_rows = [&](int32_t s, int32_t n, bool whole) {
+        if (whole && !FILL) {
+            cnt += n;
+            return;
+        }
+        for (int32_t p = s; p < s + n; p++) {
+            T x[D];
+            load_pt<T, D>(tv.pts, p, x);
+            const T d = dist2<T, D, PER>(x, q, tv.L);
+            if (whole || d <= r2) {
+                if (FILL) {
+                    dout[w] = d;  // squared, see ball_one
+                    iout[w] = load_idx(tv.idx, tv.idx64, p) - idx_base;
+                    w++;
+                }
+                cnt++;
+            }
+        }
+    };
+    // one subtree (tight box b, pair (a, n)): the wide id to descend into, or -1
+    auto visit = [&](const T *b, int32_t a, int32_t n) -> int32_t {
+        if (mindist2<T, D, PER>(b, q, tv.L) > r2) return -1;
+        if (maxdist2<T, D>(b, q) <= r2) {
+            const int2 rr = n >= 0 ? make_int2(a, n) : tv.wrange[a];
+            take_rows(rr.x, rr.y, true);
+            return -1;
+        }
+        if (n >= 0) {
+            take_rows(a, n, false);
+            return -1;
+        }
+        return a;
+    };
+    const int32_t top = visit(rootbox, root.x, root.y);
+    if (top < 0) return cnt;
+    int32_t cur = top, from = -1, prev = -1;  // from: -1 arriving from above, 0/1 back from that child
+    for (int64_t step = 0;; step++) {
+        if (step > step_limit) {
+            bad = true;
+            break;
+        }
+        T b0[2 * D], b1[2 * D];
+        int4 r;
+        load_wide<T, D>(tv.wide, cur, b0, b1, r);
+        if (from == -2) from = (r.y < 0 && r.x == prev) ? 0 : 1;  // back from a child: which one
+        int32_t next = -1;
+        if (from < 0) next = visit(b0, r.x, r.y);
+        if (next < 0 && from < 1) next = visit(b1, r.z, r.w);
+        if (next >= 0) {
+            cur = next;
+            from = -1;
+            continue;
+        }
+        if (cur == top) break;
+        prev = cur;
+        cur = tv.wparent[cur];
+        from = -2;
+    }
+    return cnt;
+}
+
 // ------------------------------------------------------------------
 // Ball search with one walk per wave (queries in leaf order), over the wide
 // records: the same decisions as ball_one for every lane -- a subtree whose

@@ -248,14 +248,15 @@ int ckd_tree_load(const char *path, void *stream, ckd_tree **out);
  *
  * ckd_batch_status: waits for `stream`, then reports the ball total and
  * whether it exceeded ball_capacity (rows past the capacity are not
- * written).
+ * written).  It returns CKD_ERR_HIP if a ball walk stopped at its step
+ * bound, which only an inconsistent tree can cause.
  */
 typedef struct ckd_batch ckd_batch;
 int ckd_batch_plan(const ckd_tree *forest, const int64_t *qoff, const int32_t *k, const double *radius,
                    int32_t sorted, int32_t threads, int32_t blocks, int64_t ball_capacity, ckd_batch **out,
                    int64_t *knn_total);
 /* kNN kernel of a plan: CKD_KNN_STACK (the default when the queries are at
-   least half as many as the points; queries sorted by leaf
+   least 1/20 as many as the points; queries sorted by leaf
    within each set, queries_per_wave in {1, 4, 16, 64} share a wave's walk,
    LDS stack of stack_depth = 32 or 64 entries, 0 = the smallest that holds
    the forest's depth; a stack shorter than the depth is never used) or
