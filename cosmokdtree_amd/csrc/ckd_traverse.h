@@ -874,17 +874,24 @@ __device__ __forceinline__ int64_t ball_wide_one(const TreeView<T> &tv, const T 
             cnt += n;
             return;
         }
-        for (int32_t p = s; p < s + n; p++) {
-            T x[D];
-            load_pt<T, D>(tv.pts, p, x);
-            const T d = dist2<T, D, PER>(x, q, tv.L);
-            if (whole || d <= r2) {
-                if (FILL) {
-                    dout[w] = d;  // squared, see ball_one
-                    iout[w] = load_idx(tv.idx, tv.idx64, p) - idx_base;
-                    w++;
+        constexpr int RB = 4;  // rows whose loads are issued together
+        for (int32_t p0 = s; p0 < s + n; p0 += RB) {
+            T x[RB][D];
+#pragma unroll
+            for (int u = 0; u < RB; u++) load_pt<T, D>(tv.pts, p0 + u < s + n ? p0 + u : s, x[u]);
+#pragma unroll
+            for (int u = 0; u < RB; u++) {
+                const int32_t p = p0 + u;
+                if (p >= s + n) break;
+                const T d = dist2<T, D, PER>(x[u], q, tv.L);
+                if (whole || d <= r2) {
+                    if (FILL) {
+                        dout[w] = d;  // squared, see ball_one
+                        iout[w] = load_idx(tv.idx, tv.idx64, p) - idx_base;
+                        w++;
+                    }
+                    cnt++;
                 }
-                cnt++;
             }
         }
     };
