@@ -874,7 +874,7 @@ __device__ __forceinline__ int64_t ball_wide_one(const TreeView<T> &tv, const T 
             cnt += n;
             return;
         }
-        constexpr int RB = 4;  // rows whose loads are issued together
+        constexpr int RB = 8;  // rows whose loads are issued together
         for (int32_t p0 = s; p0 < s + n; p0 += RB) {
             T x[RB][D];
 #pragma unroll
