@@ -1024,18 +1024,24 @@ __device__ __forceinline__ int64_t ball_stack(const TreeView<T> &tv, int2 root, 
     bool go = __any(run);
     while (go) {
         if (b >= 0) {  // a leaf some lanes only partly cover: test its rows
+            constexpr int PB = sizeof(T) == 4 ? 8 : 4;  // rows per load batch (within the row padding)
 #pragma unroll 1
-            for (int32_t p = a; p < a + b; p++) {
-                T x[D];
-                load_pt<T, D>(tv.pts, p, x);
-                const T d = dist2<T, D, PER>(x, q, tv.L);
-                if (run && d <= r2) {
-                    if (FILL) {
-                        dout[w] = d;  // squared, see ball_one
-                        iout[w] = load_idx(tv.idx, tv.idx64, p) - idx_base;
-                        w++;
+            for (int32_t p0 = a; p0 < a + b; p0 += PB) {
+                T x[PB][D];
+                load_rows<T, D, PB>(tv.pts, p0, x);
+#pragma unroll
+                for (int j = 0; j < PB; j++) {
+                    const int32_t p = p0 + j;
+                    if (p >= a + b) break;
+                    const T d = dist2<T, D, PER>(x[j], q, tv.L);
+                    if (run && d <= r2) {
+                        if (FILL) {
+                            dout[w] = d;  // squared, see ball_one
+                            iout[w] = load_idx(tv.idx, tv.idx64, p) - idx_base;
+                            w++;
+                        }
+                        cnt++;
                     }
-                    cnt++;
                 }
             }
         } else {
