@@ -56,6 +56,39 @@ def test_batch_matches_per_set_oracle(ckd, workload, kernel, qpw, leaf):
     forest.free()
 
 
+@pytest.mark.parametrize("periodic,dtype", [(False, np.float32), (True, np.float64), (False, np.float64)])
+@pytest.mark.parametrize("kernel", ["stack", "lane"])
+def test_batch_open_and_f64_forests(ckd, workload, periodic, dtype, kernel):
+    """The other compiled variants of the engine (non-periodic forests,
+    float64 coordinates): the same per-set comparison with the oracle."""
+    from cosmokdtree_amd import batch
+    sets, nq, ks, radii, qs = workload
+    sets = [s.astype(dtype) for s in sets]
+    qs = [q.astype(dtype) for q in qs]
+    box = np.full(3, L) if periodic else None
+    forest = batch.build_forest(sets, leaf=16, boxsize=None if box is None else list(box))
+    try:
+        plan = batch.BatchPlan(forest, nq, ks, radii, sorted=True, threads=128, knn_kernel=kernel,
+                               queries_per_wave=64)
+        out = plan.run(torch.from_numpy(np.concatenate(qs)).cuda())
+        off = out["ball_off"].cpu().numpy()
+        bd, bi = out["ball_dist"].cpu().numpy(), out["ball_idx"].cpu().numpy()
+        assert bd.dtype == dtype
+        for r, (pts, q) in enumerate(zip(sets, qs)):
+            orc = OracleTree(pts, 16, box)
+            k = int(ks[r])
+            d_ref, i_ref = orc.knn(q, k, True)
+            d_got, i_got = plan.knn_rows(r)
+            assert_knn_rows(pts, q, k, d_ref, i_ref, d_got.cpu().numpy(), i_got.cpu().numpy(), L=box)
+            o_ref, bd_ref, bi_ref = orc.ball_csr(q, radii[r], False)
+            q0 = int(plan.qoff[r])
+            rows = off[q0:q0 + len(q) + 1]
+            assert_csr_equal(o_ref, bd_ref, bi_ref, rows - rows[0], bd[rows[0]:rows[-1]], bi[rows[0]:rows[-1]])
+            orc.close()
+    finally:
+        forest.free()
+
+
 def test_batch_graph_replays(ckd, workload):
     from cosmokdtree_amd import batch
     sets, nq, ks, radii, qs = workload
