@@ -825,13 +825,21 @@ __device__ __forceinline__ int64_t ball_one(const TreeView<T> &tv, const T (&q)[
                     cnt += r.count;
                 } else if (nr.axis < 0) {
                     const Range r = tv.range[cur];
-                    for (int64_t p = r.start; p < r.start + r.count; p++) {
-                        T x[D];
-                        load_pt<T, D>(tv.pts, p, x);
-                        const T d = dist2<T, D, PER>(x, q, tv.L);
-                        if (d <= r2) {
-                            emit(p, d);
-                            cnt++;
+                    const int64_t pe = r.start + r.count;
+                    constexpr int RB = 8;  // rows whose loads are issued together
+                    for (int64_t p0 = r.start; p0 < pe; p0 += RB) {
+                        T x[RB][D];
+#pragma unroll
+                        for (int u = 0; u < RB; u++) load_pt<T, D>(tv.pts, p0 + u < pe ? p0 + u : r.start, x[u]);
+#pragma unroll
+                        for (int u = 0; u < RB; u++) {
+                            const int64_t p = p0 + u;
+                            if (p >= pe) break;
+                            const T d = dist2<T, D, PER>(x[u], q, tv.L);
+                            if (d <= r2) {
+                                emit(p, d);
+                                cnt++;
+                            }
                         }
                     }
                 } else {
@@ -1120,15 +1128,23 @@ __device__ __forceinline__ int64_t box_one(const TreeView<T> &tv, const T (&bx)[
                 cnt += r.count;
             } else if (nr.axis < 0) {
                 const Range r = tv.range[cur];
-                for (int64_t p = r.start; p < r.start + r.count; p++) {
-                    T x[D];
-                    load_pt<T, D>(tv.pts, p, x);
-                    bool in = true;
+                const int64_t pe = r.start + r.count;
+                constexpr int RB = 8;  // rows whose loads are issued together
+                for (int64_t p0 = r.start; p0 < pe; p0 += RB) {
+                    T x[RB][D];
 #pragma unroll
-                    for (int d = 0; d < D; d++) in = in && !(x[d] < bx[2 * d] || x[d] > bx[2 * d + 1]);
-                    if (in) {
-                        if (FILL) iout[w++] = load_idx(tv.idx, tv.idx64, p) - idx_base;
-                        cnt++;
+                    for (int u = 0; u < RB; u++) load_pt<T, D>(tv.pts, p0 + u < pe ? p0 + u : r.start, x[u]);
+#pragma unroll
+                    for (int u = 0; u < RB; u++) {
+                        const int64_t p = p0 + u;
+                        if (p >= pe) break;
+                        bool in = true;
+#pragma unroll
+                        for (int d = 0; d < D; d++) in = in && !(x[u][d] < bx[2 * d] || x[u][d] > bx[2 * d + 1]);
+                        if (in) {
+                            if (FILL) iout[w++] = load_idx(tv.idx, tv.idx64, p) - idx_base;
+                            cnt++;
+                        }
                     }
                 }
             } else {
