@@ -183,7 +183,7 @@ __global__ __launch_bounds__(QT, (sizeof(T) == 4 && KMAX == 32 && GC == 1) ? 5 :
                                               const int32_t *order, const int32_t *leaf_of, T *dout,
                                               int64_t *iout, int64_t *stats, Retry rt, int Grt) {
     const int G = GC ? GC : Grt;
-    constexpr int CB = KMAX < 16 ? 0 : (sizeof(T) == 4 ? ((KMAX == 32 && GC == 1) ? 8 : 16) : 8);
+    constexpr int CB = KMAX < 16 ? 0 : (sizeof(T) == 4 ? ((KMAX <= 32 && GC == 1) ? 8 : 16) : 8);
     __shared__ T buf_d[CB > 0 ? CB * QT : 1];
     __shared__ int32_t buf_s[CB > 0 ? CB * QT : 1];
     __shared__ StackEntry<T, D> stk[QT / WAVE][SD];
