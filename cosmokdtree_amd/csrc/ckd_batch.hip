@@ -15,7 +15,7 @@ extern "C" {
 
 int ckd_batch_free(ckd_batch *b) {
     if (!b) return CKD_OK;
-    void *ps[] = {b->items_a, b->items_b, b->kset,     b->rset,     b->qoff,     b->knn_off,   b->counters,
+    void *ps[] = {b->items_a, b->items_b, b->items_c, b->kset, b->rset, b->qoff, b->knn_off, b->counters,
                   b->counts,  b->scan_tmp, b->leaf_of, b->rank, b->order, b->leaf_cnt, b->leaf_off,
                   b->setpos,  b->order_tmp};
     for (void *p : ps)
@@ -45,7 +45,7 @@ int ckd_batch_plan(const ckd_tree *t, const int64_t *qoff, const int32_t *k, con
         return CKD_ERR_ARG;
     }
     std::vector<int64_t> knn_off(ns + 1, 0);
-    std::vector<BatchItem> a, bb;
+    std::vector<BatchItem> a, bb, cc;
     bool any_ball = false;
     for (int64_t r = 0; r < ns; r++) {
         const int64_t nq = qoff[r + 1] - qoff[r];
@@ -68,17 +68,14 @@ int ckd_batch_plan(const ckd_tree *t, const int64_t *qoff, const int32_t *k, con
             }
             if (ball) {
                 w.kind = 1;
-                a.push_back(w);
+                cc.push_back(w);
                 w.kind = 2;
                 bb.push_back(w);
             }
         }
     }
-    // heaviest items first: kNN by descending k, then ball counts
-    std::stable_sort(a.begin(), a.end(), [&](const BatchItem &x, const BatchItem &y) {
-        const int kx = x.kind == 0 ? k[x.set] : -1, ky = y.kind == 0 ? k[y.set] : -1;
-        return kx > ky;
-    });
+    // heaviest kNN items first (descending k)
+    std::stable_sort(a.begin(), a.end(), [&](const BatchItem &x, const BatchItem &y) { return k[x.set] > k[y.set]; });
     ckd_batch *b = new ckd_batch();
     b->tree = t;
     b->nsets = ns;
@@ -99,6 +96,7 @@ int ckd_batch_plan(const ckd_tree *t, const int64_t *qoff, const int32_t *k, con
     b->grid = blocks > 0 ? blocks : dev_cu * (2048 / threads);
     b->n_a = (int64_t)a.size();
     b->n_b = (int64_t)bb.size();
+    b->n_c = (int64_t)cc.size();
     std::vector<double> rs(ns, 0.0);
     if (radius)
         for (int64_t r = 0; r < ns; r++) rs[r] = radius[r];
@@ -110,12 +108,16 @@ int ckd_batch_plan(const ckd_tree *t, const int64_t *qoff, const int32_t *k, con
     hipError_t e = hipSuccess;
     if (e == hipSuccess) e = up((void **)&b->items_a, a.data(), sizeof(BatchItem) * a.size());
     if (e == hipSuccess) e = up((void **)&b->items_b, bb.data(), sizeof(BatchItem) * bb.size());
+    if (e == hipSuccess) e = up((void **)&b->items_c, cc.data(), sizeof(BatchItem) * cc.size());
     if (e == hipSuccess) e = up((void **)&b->kset, k, sizeof(int32_t) * ns);
     if (e == hipSuccess) e = up((void **)&b->rset, rs.data(), sizeof(double) * ns);
     if (e == hipSuccess) e = up((void **)&b->qoff, qoff, sizeof(int64_t) * (ns + 1));
     if (e == hipSuccess) e = up((void **)&b->knn_off, knn_off.data(), sizeof(int64_t) * (ns + 1));
     if (e == hipSuccess) e = up((void **)&b->counters, nullptr, sizeof(int64_t) * 4);
     if (e == hipSuccess) e = up((void **)&b->counts, nullptr, sizeof(int64_t) * (b->nq > 0 ? b->nq : 1));
+    // queries of sets without a ball search keep a zero count: only the
+    // ball-count items write counts
+    if (e == hipSuccess) e = hipMemset(b->counts, 0, sizeof(int64_t) * (b->nq > 0 ? b->nq : 1));
     if (e == hipSuccess) e = up(&b->scan_tmp, nullptr, scan_tmp_bytes(b->nq));
     if (b->nq > 0) {  // leaf ordering of the queries (stack kNN walk, query order of the other walks)
         const size_t nqb = (size_t)(b->nq > 0 ? b->nq : 1);

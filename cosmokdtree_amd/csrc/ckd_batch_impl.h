@@ -39,8 +39,9 @@ struct ckd_batch {
     int64_t nsets = 0, nq = 0, knn_total = 0, capacity = 0;
     int sorted = 1, nt = 256, grid = 1024, has_ball = 0, has_knn = 0;
     int knn_kernel = CKD_KNN_STACK, qpw = 64, stack = 0;  // set by ckd_batch_plan / ckd_batch_config
-    BatchItem *items_a = nullptr, *items_b = nullptr;
-    int64_t n_a = 0, n_b = 0;
+    // items_a: kNN (k_batch); items_c: ball counts, items_b: ball fills (k_batch_ball)
+    BatchItem *items_a = nullptr, *items_b = nullptr, *items_c = nullptr;
+    int64_t n_a = 0, n_b = 0, n_c = 0;
     int32_t *kset = nullptr;
     double *rset = nullptr;
     int64_t *qoff = nullptr, *knn_off = nullptr, *counters = nullptr, *counts = nullptr;
@@ -58,6 +59,7 @@ __global__ void k_batch_reset(int64_t *counters) {
     counters[0] = 0;  // phase A head
     counters[1] = 0;  // phase B head
     counters[2] = 0;  // ball overflow flag
+    counters[3] = 0;  // ball-count head
 }
 
 // ---- query ordering for the stack kNN path: each query's set (binary
@@ -143,12 +145,10 @@ __device__ __forceinline__ void batch_knn_stack(const TreeView<T> &tv, const Bat
 template <typename T, int D, bool PER, int NT, int SD, int GC>
 __global__ __launch_bounds__(NT, 3) void k_batch(TreeView<T> tv, const int64_t *set_off, const T *queries,
                                               const BatchItem *items, int64_t nitems, int64_t *head,
-                                              const int32_t *kset, const double *rset, const int64_t *qoff,
+                                              const int32_t *kset, const int64_t *qoff,
                                               const int64_t *knn_off, int sorted, T *kdist, int64_t *kidx,
-                                              int64_t *ball_counts, const int64_t *ball_off, T *bdist,
-                                              int64_t *bidx, int64_t capacity, int64_t *overflow,
                                               const int32_t *order, const int32_t *leaf_of, int use_stack,
-                                              int Grt, int64_t step_limit) {
+                                              int Grt) {
     const int G = GC ? GC : Grt;
     __shared__ int64_t s_item;
     constexpr int CBMAX = sizeof(T) == 4 ? 16 : 8;
@@ -189,7 +189,7 @@ __global__ __launch_bounds__(NT, 3) void k_batch(TreeView<T> tv, const int64_t *
         for (int d = 0; d < D; d++) q[d] = queries[qg * D + d];
         const int32_t root = w.set;
         const int64_t base = set_off[w.set];
-        if (w.kind == 0) {
+        {  // one kNN walk per query
             const int k = kset[w.set];
             const int64_t local = qg - qoff[w.set];
             T *drow = kdist + knn_off[w.set] + local * k;
@@ -200,30 +200,57 @@ __global__ __launch_bounds__(NT, 3) void k_batch(TreeView<T> tv, const int64_t *
             else if (k <= 16) knn_one<T, D, PER, 16>(tv, q, leaf, k, sorted, T(FLT_MAX), drow, irow, base, c);
             else if (k <= 32) knn_one<T, D, PER, 32>(tv, q, leaf, k, sorted, T(FLT_MAX), drow, irow, base, c);
             else knn_one<T, D, PER, 64>(tv, q, leaf, k, sorted, T(FLT_MAX), drow, irow, base, c);
-        } else {
-            const T r = (T)rset[w.set];
-            const T r2 = r * r;
-            if (tv.wparent) {  // over the wide records (present when rows fit int32)
-                const int2 rp = tv.wroot[w.set];
-                const T *rb = tv.dbox + (int64_t)w.set * 2 * D;
-                bool bad = false;
-                if (w.kind == 1) {
-                    ball_counts[qg] = ball_wide_one<T, D, PER, false>(tv, q, rp, rb, r2, nullptr, nullptr, base,
-                                                                      step_limit, bad);
-                } else {
-                    const int64_t o = ball_off[qg];
-                    if (ball_off[qg + 1] <= capacity)
-                        ball_wide_one<T, D, PER, true>(tv, q, rp, rb, r2, bdist + o, bidx + o, base, step_limit, bad);
-                    else atomicOr((unsigned long long *)overflow, 1ull);
-                }
-                if (bad) atomicOr((unsigned long long *)overflow, 2ull);
-            } else if (w.kind == 1) {
-                ball_counts[qg] = ball_one<T, D, PER, false>(tv, q, root, r2, nullptr, nullptr, base);
+        }
+    }
+}
+
+// The ball items (kind 1: count, kind 2: fill), one walk per query, in a
+// persistent kernel of their own: without the kNN lists it needs far fewer
+// registers than k_batch, so more waves hide the walks' load latency.
+template <typename T, int D, bool PER, int NT>
+__global__ __launch_bounds__(NT) void k_batch_ball(TreeView<T> tv, const int64_t *set_off, const T *queries,
+                                                   const BatchItem *items, int64_t nitems, int64_t *head,
+                                                   const double *rset, const int32_t *order, int64_t *ball_counts,
+                                                   const int64_t *ball_off, T *bdist, int64_t *bidx,
+                                                   int64_t capacity, int64_t *overflow, int64_t step_limit) {
+    __shared__ int64_t s_item;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = (int64_t)atomicAdd((unsigned long long *)head, 1ull);
+        __syncthreads();
+        const int64_t it = s_item;
+        __syncthreads();
+        if (it >= nitems) break;
+        const BatchItem w = items[it];
+        if ((int)threadIdx.x >= w.nq) continue;
+        // the item's queries in the set's leaf order, so neighbouring lanes
+        // walk the same nodes
+        const int64_t qg = order ? (int64_t)order[w.q0 + threadIdx.x] : w.q0 + threadIdx.x;
+        T q[D];
+#pragma unroll
+        for (int d = 0; d < D; d++) q[d] = queries[qg * D + d];
+        const int64_t base = set_off[w.set];
+        const T r = (T)rset[w.set];
+        const T r2 = r * r;
+        if (tv.wparent) {  // over the wide records (present when rows fit int32)
+            const int2 rp = tv.wroot[w.set];
+            const T *rb = tv.dbox + (int64_t)w.set * 2 * D;
+            bool bad = false;
+            if (w.kind == 1) {
+                ball_counts[qg] = ball_wide_one<T, D, PER, false>(tv, q, rp, rb, r2, nullptr, nullptr, base,
+                                                                  step_limit, bad);
             } else {
                 const int64_t o = ball_off[qg];
-                if (ball_off[qg + 1] <= capacity) ball_one<T, D, PER, true>(tv, q, root, r2, bdist + o, bidx + o, base);
+                if (ball_off[qg + 1] <= capacity)
+                    ball_wide_one<T, D, PER, true>(tv, q, rp, rb, r2, bdist + o, bidx + o, base, step_limit, bad);
                 else atomicOr((unsigned long long *)overflow, 1ull);
             }
+            if (bad) atomicOr((unsigned long long *)overflow, 2ull);
+        } else if (w.kind == 1) {
+            ball_counts[qg] = ball_one<T, D, PER, false>(tv, q, w.set, r2, nullptr, nullptr, base);
+        } else {
+            const int64_t o = ball_off[qg];
+            if (ball_off[qg + 1] <= capacity) ball_one<T, D, PER, true>(tv, q, w.set, r2, bdist + o, bidx + o, base);
+            else atomicOr((unsigned long long *)overflow, 1ull);
         }
     }
 }
@@ -252,30 +279,28 @@ int run_t(ckd_batch *b, const T *queries, T *kdist, int64_t *kidx, int64_t *ball
     }
     const int G = WAVE / b->qpw;
     const int64_t step_limit = 3 * t->nwide + 3;  // bound of the ball walk over the wide records
-    if (b->n_a > 0) {
+    if (b->n_a > 0) {  // kNN items
         if (G == 1 || !use_stack)
             k_batch<T, D, PER, NT, SD, 1><<<b->grid, NT, 0, s>>>(tv, t->set_off, queries, b->items_a, b->n_a,
-                                                                 b->counters, b->kset, b->rset, b->qoff, b->knn_off,
-                                                                 b->sorted, kdist, kidx, b->counts, ball_off, bdist,
-                                                                 bidx, b->capacity, b->counters + 2, order, b->leaf_of, use_stack, 1,
-                                                                 step_limit);
+                                                                 b->counters, b->kset, b->qoff, b->knn_off, b->sorted,
+                                                                 kdist, kidx, order, b->leaf_of, use_stack, 1);
         else
             k_batch<T, D, PER, NT, SD, 0><<<b->grid, NT, 0, s>>>(tv, t->set_off, queries, b->items_a, b->n_a,
-                                                                 b->counters, b->kset, b->rset, b->qoff, b->knn_off,
-                                                                 b->sorted, kdist, kidx, b->counts, ball_off, bdist,
-                                                                 bidx, b->capacity, b->counters + 2, order, b->leaf_of, use_stack, G,
-                                                                 step_limit);
+                                                                 b->counters, b->kset, b->qoff, b->knn_off, b->sorted,
+                                                                 kdist, kidx, order, b->leaf_of, use_stack, G);
     }
     CKD_HIP(hipGetLastError());
     if (b->has_ball) {
+        if (b->n_c > 0)  // ball counts
+            k_batch_ball<T, D, PER, NT><<<b->grid, NT, 0, s>>>(tv, t->set_off, queries, b->items_c, b->n_c,
+                                                               b->counters + 3, b->rset, order, b->counts, ball_off,
+                                                               bdist, bidx, b->capacity, b->counters + 2, step_limit);
         int rc = scan_i64(b->counts, ball_off, b->nq, true, b->scan_tmp, s);
         if (rc) return rc;
-        if (b->n_b > 0) {
-            k_batch<T, D, PER, NT, SD, 1><<<b->grid, NT, 0, s>>>(tv, t->set_off, queries, b->items_b, b->n_b,
-                                                                 b->counters + 1, b->kset, b->rset, b->qoff,
-                                                                 b->knn_off, b->sorted, kdist, kidx, b->counts,
-                                                                 ball_off, bdist, bidx, b->capacity, b->counters + 2,
-                                                                 order, b->leaf_of, 0, 1, step_limit);
+        if (b->n_b > 0) {  // ball fills
+            k_batch_ball<T, D, PER, NT><<<b->grid, NT, 0, s>>>(tv, t->set_off, queries, b->items_b, b->n_b,
+                                                               b->counters + 1, b->rset, order, b->counts, ball_off,
+                                                               bdist, bidx, b->capacity, b->counters + 2, step_limit);
             // the fill walks wrote d^2; rows past the capacity were not written
             k_sqrt_rows<T><<<SQRT_BLOCKS, 256, 0, s>>>(bdist, ball_off + b->nq, b->capacity);
         }

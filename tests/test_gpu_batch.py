@@ -89,6 +89,43 @@ def test_batch_open_and_f64_forests(ckd, workload, periodic, dtype, kernel):
         forest.free()
 
 
+def test_batch_sets_without_ball(ckd, workload):
+    """Sets with radius 0 (and k = 0) take part without a ball search: their
+    ball rows are empty and the other sets' rows still match the oracle."""
+    from cosmokdtree_amd import batch
+    sets, nq, ks, radii, qs = workload
+    radii = radii.copy()
+    radii[::3] = 0.0
+    ks = ks.copy()
+    ks[1::5] = 0
+    forest = batch.build_forest(sets, leaf=16, boxsize=[L] * 3)
+    try:
+        for rep in range(2):  # the second run reuses the plan's buffers
+            plan = batch.BatchPlan(forest, nq, ks, radii, sorted=True, threads=128, knn_kernel="stack",
+                                   queries_per_wave=64)
+            out = plan.run(torch.from_numpy(np.concatenate(qs)).cuda())
+            off = out["ball_off"].cpu().numpy()
+            bd, bi = out["ball_dist"].cpu().numpy(), out["ball_idx"].cpu().numpy()
+            for r, (pts, q) in enumerate(zip(sets, qs)):
+                q0 = int(plan.qoff[r])
+                rows = off[q0:q0 + len(q) + 1]
+                if radii[r] == 0:
+                    assert rows[-1] == rows[0], f"set {r} has no ball search but {rows[-1] - rows[0]} rows"
+                    continue
+                orc = OracleTree(pts, 16, np.full(3, L))
+                o_ref, bd_ref, bi_ref = orc.ball_csr(q, radii[r], False)
+                assert_csr_equal(o_ref, bd_ref, bi_ref, rows - rows[0], bd[rows[0]:rows[-1]], bi[rows[0]:rows[-1]])
+                if ks[r] > 0:
+                    d_ref, i_ref = orc.knn(q, int(ks[r]), True)
+                    d_got, i_got = plan.knn_rows(r)
+                    assert_knn_rows(pts, q, int(ks[r]), d_ref, i_ref, d_got.cpu().numpy(), i_got.cpu().numpy(),
+                                    L=np.full(3, L))
+                orc.close()
+            del plan
+    finally:
+        forest.free()
+
+
 def test_batch_graph_replays(ckd, workload):
     from cosmokdtree_amd import batch
     sets, nq, ks, radii, qs = workload
