@@ -8,11 +8,12 @@
 // kNN items run the stack walk from the set's root (whole waves, queries in
 // leaf order within their set, 64 / G queries per wave; ckd_batch_config)
 // or one walk per query;
-// phase A holds the kNN items (largest k first) and the ball-count items,
-// phase B the ball-fill items.  Each launch starts one grid of NT-thread
-// blocks that pull items from an atomic head counter until the queue is
-// empty, so sets of very different sizes, k and radii share one launch.
-// Between the phases a device scan turns the ball counts into CSR offsets.
+// the kNN items (largest k first) run in k_batch, the ball-count items and
+// then the ball-fill items in k_batch_ball (one walk per query, few
+// registers).  Each launch starts one grid of NT-thread blocks that pull
+// items from an atomic head counter until its queue is empty, so sets of
+// very different sizes, k and radii share a launch.  Between the ball
+// counts and fills a device scan turns the counts into CSR offsets.
 // ckd_batch_run only launches kernels (no allocation, copy or host wait), so
 // a caller can capture it in a hipGraph and replay it.
 //
