@@ -64,6 +64,22 @@ def build_forest(sets, leaf: int = 16, boxsize=None, dtype=None) -> KDTree:
     return tree
 
 
+INT32_MAX = 2 ** 31 - 1
+
+
+def check_plan_limits(n_points: int, qoff) -> None:
+    """The batch kernels hold rows, query positions and leaf ranks in 32
+    bits: forests and query batches must stay below 2^31 - 1 rows (the C
+    plan rejects them too, with CKD_ERR_LIMIT)."""
+    qoff = np.asarray(qoff, dtype=np.int64)
+    if n_points >= INT32_MAX:
+        raise ValueError(f"the batched engine takes forests below 2^31 - 1 points (got {n_points})")
+    if len(qoff) and qoff[-1] >= INT32_MAX:
+        raise ValueError(f"the batched engine takes batches below 2^31 - 1 queries (got {int(qoff[-1])})")
+    if len(qoff) and (qoff[0] != 0 or np.any(np.diff(qoff) < 0)):
+        raise ValueError("query counts must not be negative")
+
+
 class BatchPlan:
     """Work queue and output layout for one batched query workload."""
 
@@ -79,6 +95,9 @@ class BatchPlan:
         self.radius = None if radius_per_set is None else np.ascontiguousarray(
             np.asarray(radius_per_set, dtype=np.float64))
         self.nq = int(self.qoff[-1])
+        check_plan_limits(forest.n, self.qoff)
+        # a set takes part in the ball search only with a positive radius
+        self.has_ball = self.radius is not None and bool(np.any(self.radius > 0))
         self.sorted = bool(sorted)
         self.threads, self.blocks = int(threads), int(blocks)
         self.ball_capacity = ball_capacity
@@ -87,8 +106,11 @@ class BatchPlan:
         self._make(ball_capacity or 0)
 
     def _make(self, capacity):
+        # a captured graph holds the old plan's buffers: it must not run again
+        self._graph = None
         if self._plan:
             lib.ckd_batch_free(self._plan)
+            self._plan = None
         plan = C.c_void_p()
         tot = C.c_int64(0)
         check(lib.ckd_batch_plan(self.forest._ptr, self.qoff.ctypes.data_as(C.c_void_p),
@@ -103,7 +125,8 @@ class BatchPlan:
         dev, dt = self.forest.device, self.forest.torch_dtype
         self.knn_dist = torch.empty(max(self.knn_total, 1), dtype=dt, device=dev)
         self.knn_idx = torch.empty(max(self.knn_total, 1), dtype=torch.int64, device=dev)
-        self.ball_off = torch.empty(self.nq + 1, dtype=torch.int64, device=dev)
+        # zeros: the kernels write the offsets only when some set has a ball search
+        self.ball_off = torch.zeros(self.nq + 1, dtype=torch.int64, device=dev)
         self.ball_dist = torch.empty(max(self.capacity, 1), dtype=dt, device=dev)
         self.ball_idx = torch.empty(max(self.capacity, 1), dtype=torch.int64, device=dev)
 
@@ -148,7 +171,7 @@ class BatchPlan:
         """Eager run; grows the ball capacity and reruns once if it was short."""
         queries = queries.to(device=self.forest.device, dtype=self.forest.torch_dtype).contiguous()
         self.launch(queries)
-        if self.radius is not None:
+        if self.has_ball:
             tot, ovf = self.status()
             if ovf:
                 self._make(tot)
@@ -156,7 +179,7 @@ class BatchPlan:
         return self.results()
 
     def results(self):
-        tot = int(self.ball_off[-1].item()) if self.radius is not None else 0
+        tot = int(self.ball_off[-1].item()) if self.has_ball else 0
         return dict(knn_dist=self.knn_dist[:self.knn_total], knn_idx=self.knn_idx[:self.knn_total],
                     ball_off=self.ball_off, ball_dist=self.ball_dist[:tot], ball_idx=self.ball_idx[:tot])
 
@@ -177,7 +200,19 @@ class BatchPlan:
         return g
 
     def replay(self):
+        """Replay the captured launches.  The graph writes the buffers the plan
+        had at capture time; a ball total beyond their capacity (queries
+        changed in place since the capture) raises, as does a replay after the
+        plan was re-made (by a run that grew the capacity, or a config change):
+        capture again then."""
+        if self._graph is None:
+            raise RuntimeError("no valid captured graph: call capture() (again) first")
         self._graph.replay()
+        if self.has_ball:
+            tot, ovf = self.status()
+            if ovf:
+                raise RuntimeError(f"ball results ({tot}) exceed the captured capacity ({self.capacity}): "
+                                   "run() and capture() again")
         return self.results()
 
     def knn_rows(self, r: int):

@@ -44,6 +44,12 @@ int ckd_batch_plan(const ckd_tree *t, const int64_t *qoff, const int32_t *k, con
         set_error("query offsets must start at 0");
         return CKD_ERR_ARG;
     }
+    // the batch kernels hold rows, query positions and leaf ranks in 32 bits
+    if (t->n >= INT32_MAX || qoff[ns] >= INT32_MAX || t->nnodes >= INT32_MAX) {
+        set_error("the batched engine takes forests and query batches below 2^31 - 1 rows "
+                  "(use ckd_knn / ckd_ball_* per tree beyond)");
+        return CKD_ERR_LIMIT;
+    }
     std::vector<int64_t> knn_off(ns + 1, 0);
     std::vector<BatchItem> a, bb, cc;
     bool any_ball = false;

@@ -318,7 +318,7 @@ int run_nt(ckd_batch *b, const void *q, void *kd, int64_t *ki, int64_t *bo, void
     }
     // a stack shorter than the tree is never used: 64 entries if 32 do not
     // fit, one walk per query if 64 do not
-    if (b->tree->depth > 64 && b->knn_kernel == CKD_KNN_STACK) b->knn_kernel = CKD_KNN_LANE;
+    if ((b->tree->depth > 64 || !b->tree->wide) && b->knn_kernel == CKD_KNN_STACK) b->knn_kernel = CKD_KNN_LANE;
     const bool sd32 = b->tree->depth <= 32 && b->stack != 64;
     switch (b->nt) {
         case 64:

@@ -47,6 +47,113 @@ std::vector<Array> arrays(ckd_tree *t) {
     };
 }
 
+// Checks of one loaded array before it goes to the device: every index a
+// kernel follows must stay inside its array, whatever the file holds.
+// Returns an empty string when the array is consistent.  `depth` receives
+// the depth recomputed from the node links (array 0).
+template <typename T>
+std::string check_nodes(const char *raw, const PersistHeader &h, int32_t &depth) {
+    const NodeRec<T> *nd = (const NodeRec<T> *)raw;
+    const int64_t nn = h.nnodes;
+    int64_t leaves = 0;
+    for (int64_t i = 0; i < nn; i++) {
+        const NodeRec<T> &r = nd[i];
+        if (r.axis < 0) {
+            if (r.child != -1) return "a leaf with children";
+            leaves++;
+        } else {
+            if (r.axis >= h.ndim) return "a split axis beyond ndim";
+            if (r.child < h.nsets || (int64_t)r.child + 1 >= nn) return "a child index out of range";
+        }
+        if (r.parent < -1 || r.parent >= nn) return "a parent index out of range";
+    }
+    if (leaves != h.nleaves) return "the leaf count differs from the header";
+    // walk every set's tree from its root: each node reached once, depth counted
+    std::vector<int32_t> lvl(nn, -1);
+    std::vector<int64_t> stk;
+    int64_t seen = 0;
+    depth = 0;
+    for (int64_t r = 0; r < h.nsets; r++) {
+        if (nd[r].parent != -1) return "a set root with a parent";
+        lvl[r] = 0;
+        stk.push_back(r);
+        while (!stk.empty()) {
+            const int64_t v = stk.back();
+            stk.pop_back();
+            seen++;
+            depth = lvl[v] > depth ? lvl[v] : depth;
+            if (nd[v].axis >= 0)
+                for (int64_t c = nd[v].child; c <= (int64_t)nd[v].child + 1; c++) {
+                    if (lvl[c] >= 0 || nd[c].parent != v) return "node links do not form a tree";
+                    lvl[c] = lvl[v] + 1;
+                    stk.push_back(c);
+                }
+        }
+    }
+    if (seen != nn) return "nodes unreachable from the roots";
+    return "";
+}
+
+std::string check_pair(int2 p, int64_t nwide, int64_t n) {
+    if (p.y < 0) return p.x >= 0 && p.x < nwide ? "" : "a wide id out of range";
+    return p.x >= 0 && (int64_t)p.x + p.y <= n ? "" : "leaf rows out of range";
+}
+
+std::string check_array(int which, const char *raw, size_t bytes, const PersistHeader &h, int32_t &depth) {
+    const int64_t n = h.n, nn = h.nnodes;
+    switch (which) {
+        case 0:
+            return h.prec == 4 ? check_nodes<float>(raw, h, depth) : check_nodes<double>(raw, h, depth);
+        case 3: {
+            const Range *rg = (const Range *)raw;
+            for (int64_t i = 0; i < nn; i++)
+                if (rg[i].start < 0 || rg[i].count < 0 || rg[i].start + rg[i].count > n) return "a row range out of range";
+            return "";
+        }
+        case 5: {
+            for (int64_t i = 0; i < n; i++) {
+                const int64_t v = h.idx_bits == 64 ? ((const int64_t *)raw)[i] : ((const int32_t *)raw)[i];
+                if (v < 0 || v >= n) return "an original index out of range";
+            }
+            return "";
+        }
+        case 6: {
+            const size_t wb = wide_bytes_rt(h.ndim, h.prec);
+            for (int64_t w = 0; w < h.nwide; w++) {
+                const int4 ref = *(const int4 *)(raw + w * wb + 4 * (size_t)h.ndim * h.prec);
+                std::string e = check_pair(make_int2(ref.x, ref.y), h.nwide, n);
+                if (e.empty()) e = check_pair(make_int2(ref.z, ref.w), h.nwide, n);
+                if (!e.empty()) return e;
+            }
+            return "";
+        }
+        case 7: {
+            for (int64_t r = 0; r < h.nsets; r++) {
+                std::string e = check_pair(((const int2 *)raw)[r], h.nwide, n);
+                if (!e.empty()) return e;
+            }
+            return "";
+        }
+        case 8: {
+            for (int64_t w = 0; w < h.nwide; w++) {
+                const int2 p = ((const int2 *)raw)[w];
+                if (p.x < 0 || p.y < 0 || (int64_t)p.x + p.y > n) return "inner-node rows out of range";
+            }
+            return "";
+        }
+        case 9: {
+            const int64_t *so = (const int64_t *)raw;
+            if (so[0] != 0 || so[h.nsets] != n) return "set offsets do not span the points";
+            for (int64_t r = 0; r < h.nsets; r++)
+                if (so[r + 1] <= so[r]) return "set offsets not increasing";
+            return "";
+        }
+        default:
+            (void)bytes;
+            return "";
+    }
+}
+
 }  // namespace
 }  // namespace ckd
 
@@ -121,6 +228,16 @@ int ckd_tree_load(const char *path, void *stream, ckd_tree **out) {
         set_error("%s is not a saved cosmokdtree_amd tree", path);
         return CKD_ERR_ARG;
     }
+    bool bad = h.nnodes >= INT32_MAX || h.nsets > h.nnodes || h.nleaves < 1 || h.nleaves > h.nnodes ||
+               h.nwide < -1 || h.nwide >= h.nnodes || h.leafsize < 1 || h.depth < 0 ||
+               (h.periodic != 0 && h.periodic != 1) || (h.idx_bits == 32 && h.n >= INT32_MAX) ||
+               (h.nwide >= 0 && h.n >= INT32_MAX);
+    for (int d = 0; d < h.ndim && h.periodic; d++) bad |= !(h.L[d] > 0);
+    if (bad) {
+        fclose(f);
+        set_error("%s: inconsistent tree header", path);
+        return CKD_ERR_ARG;
+    }
     ckd_tree *t = new ckd_tree();
     t->n = h.n;
     t->nnodes = h.nnodes;
@@ -146,11 +263,19 @@ int ckd_tree_load(const char *path, void *stream, ckd_tree **out) {
     as[9].bytes = sizeof(int64_t) * (size_t)(t->nsets + 1);  // always saved
     std::vector<char> host;
     int rc = CKD_OK;
-    for (Array &a : as) {
+    int32_t depth = 0;
+    for (size_t ai = 0; ai < as.size(); ai++) {
+        Array &a = as[ai];
         if (a.bytes == 0) continue;
         host.resize(a.bytes);
         if (fread(host.data(), 1, a.bytes, f) != a.bytes) {
             set_error("%s is truncated", path);
+            rc = CKD_ERR_ARG;
+            break;
+        }
+        const std::string why = check_array((int)ai, host.data(), a.bytes, h, depth);
+        if (!why.empty()) {
+            set_error("%s: %s", path, why.c_str());
             rc = CKD_ERR_ARG;
             break;
         }
@@ -162,7 +287,13 @@ int ckd_tree_load(const char *path, void *stream, ckd_tree **out) {
             break;
         }
     }
+    if (rc == CKD_OK && fgetc(f) != EOF) {
+        set_error("%s: data after the last array", path);
+        rc = CKD_ERR_ARG;
+    }
     fclose(f);
+    // the traversals size their stacks by the depth: take the one the links give
+    t->depth = depth;
     if (rc == CKD_OK && has_wide) {  // parent pointers are derived, not stored
         rc = wide_parents(t, (hipStream_t)stream);
         if (rc == CKD_OK) {

@@ -18,9 +18,12 @@ the reference, all deliberate:
   with PERIODIC and stops the process when a periodic build gets no box,
   cosmokdtree2py.f90:48-60).  Each tree keeps its own box size (the
   reference keeps one module-global L, cosmokdtree.f90:86-88,174).
-* float32 points build a float32 tree (the reference Python path is
-  float64 only, src/Makefile:22-25; float32 reproduces a
-  DOUBLEPRECISION=0 Fortran build).  Anything else is converted to float64.
+* Points, targets, radii and boxes are converted to float64, as the
+  reference does (src/cosmokdtree/kdtree.py:31,43,59,83), whatever their
+  dtype, and distances come back as float64.  A float32 tree (what a
+  DOUBLEPRECISION=0 Fortran build computes) is built only on request:
+  ``build_kdtree(points, dtype=np.float32)``, or the device API
+  (``_build`` with a float32 tensor); its distances are float32.
 * Errors raise ValueError / MemoryError / RuntimeError instead of STOPping
   the process (cosmokdtree.f90:161,190; cosmokdtree2py.f90:58,276).
 * Sorted results list equal distances in index order (the reference's
@@ -282,8 +285,14 @@ def release_memory() -> None:
 
 # --------------------------------------------------------------- drop-in API
 
-def build_kdtree(points, leaf=None, boxsize=None):
-    """src/cosmokdtree/kdtree.py:16-34."""
+def build_kdtree(points, leaf=None, boxsize=None, *, dtype=np.float64):
+    """src/cosmokdtree/kdtree.py:16-34.  The points are converted to float64
+    (kdtree.py:31) unless the keyword dtype=np.float32 asks for a float32
+    tree (no reference counterpart: the reference Python path is float64
+    only, src/Makefile:22-25)."""
+    dtype = np.dtype(dtype)
+    if dtype not in (np.dtype(np.float64), np.dtype(np.float32)):
+        raise ValueError("dtype must be float64 (the reference's) or float32")
     if leaf is None:
         leaf = 16
     assert leaf > 0, "Leaf size must be a positive integer"
@@ -292,6 +301,9 @@ def build_kdtree(points, leaf=None, boxsize=None):
     assert len(boxsize) == points.shape[1], "Incorrect dimensions for boxsize var. Size must be [Lx, Ly, Lz, ...]"
     boxsize = np.array(boxsize, dtype=np.float64)
     periodic = bool(np.any(boxsize != -1.))  # cosmokdtree2py.f90:48-55
+    if isinstance(points, torch.Tensor):
+        points = points.detach().cpu().numpy()
+    points = np.ascontiguousarray(points, dtype=dtype)
     return _build(points, int(leaf), boxsize if periodic else None)
 
 

@@ -100,9 +100,9 @@ def test_batch_sets_without_ball(ckd, workload):
     ks[1::5] = 0
     forest = batch.build_forest(sets, leaf=16, boxsize=[L] * 3)
     try:
-        for rep in range(2):  # the second run reuses the plan's buffers
-            plan = batch.BatchPlan(forest, nq, ks, radii, sorted=True, threads=128, knn_kernel="stack",
-                                   queries_per_wave=64)
+        plan = batch.BatchPlan(forest, nq, ks, radii, sorted=True, threads=128, knn_kernel="stack",
+                               queries_per_wave=64)
+        for rep in range(2):  # the second run reuses the plan and its buffers
             out = plan.run(torch.from_numpy(np.concatenate(qs)).cuda())
             off = out["ball_off"].cpu().numpy()
             bd, bi = out["ball_dist"].cpu().numpy(), out["ball_idx"].cpu().numpy()
@@ -121,7 +121,7 @@ def test_batch_sets_without_ball(ckd, workload):
                     assert_knn_rows(pts, q, int(ks[r]), d_ref, i_ref, d_got.cpu().numpy(), i_got.cpu().numpy(),
                                     L=np.full(3, L))
                 orc.close()
-            del plan
+        del plan
     finally:
         forest.free()
 
@@ -145,3 +145,31 @@ def test_batch_graph_replays(ckd, workload):
         assert torch.equal(torch.sort(got["ball_idx"])[0], torch.sort(eager["ball_idx"])[0])
         del junk
     forest.free()
+
+
+def test_batch_replan_invalidates_graph(ckd, workload):
+    """A run that outgrows the ball capacity re-makes the plan: the graph
+    captured on the old buffers must not replay, and a plan whose sets all
+    have radius 0 reports empty ball rows."""
+    from cosmokdtree_amd import batch
+    sets, nq, ks, radii, qs = workload
+    forest = batch.build_forest(sets, leaf=16, boxsize=[L] * 3)
+    try:
+        q = torch.from_numpy(np.concatenate(qs)).cuda()
+        plan = batch.BatchPlan(forest, nq, ks, radii * 0.5, sorted=True)
+        plan.capture(q)
+        small_cap = plan.capacity
+        plan.radius = np.ascontiguousarray(radii * 1.0)
+        plan._make(small_cap)  # same capacity, larger radii: the next run overflows and re-plans
+        with pytest.raises(RuntimeError):
+            plan.replay()
+        out = plan.run(q)
+        assert plan.capacity > small_cap and plan._graph is None
+        assert int(out["ball_off"][-1]) == plan.capacity
+        with pytest.raises(RuntimeError):
+            plan.replay()
+        none = batch.BatchPlan(forest, nq, ks, np.zeros(len(nq)), sorted=True)
+        out = none.run(q)
+        assert not none.has_ball and torch.count_nonzero(out["ball_off"]) == 0 and out["ball_idx"].numel() == 0
+    finally:
+        forest.free()

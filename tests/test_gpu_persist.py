@@ -59,3 +59,38 @@ def test_forest_roundtrip(ckd, tmp_path):
     finally:
         forest.free()
         back.free()
+
+
+def test_load_recomputes_depth(ckd, tmp_path):
+    """The traversals size their LDS stacks by the tree depth: a file whose
+    header understates it loads with the depth its node links give, and
+    answers as the built tree does; a node link out of range is refused."""
+    import struct
+    from cosmokdtree_amd.synthetic import clustered_periodic
+    pts = clustered_periodic(40000, 100.0, 0.5, seed=19, dtype=np.float32)
+    tree = ckd.kdtree._build(pts, 16, np.full(3, 100.0))
+    path = tmp_path / "tree.ckd"
+    tree.save(path)
+    raw = bytearray(path.read_bytes())
+    depth_at = 8 + 5 * 8 + 5 * 4  # PersistHeader: magic, 5 int64, then ndim..idx_bits, depth
+    assert struct.unpack_from("<i", raw, depth_at)[0] == tree.info().depth
+    struct.pack_into("<i", raw, depth_at, 1)
+    bad = tmp_path / "shallow.ckd"
+    bad.write_bytes(bytes(raw))
+    back = ckd.kdtree.load_tree(bad)
+    try:
+        assert back.info().depth == tree.info().depth > 1
+        q = torch.from_numpy(pts[::5]).cuda()
+        d0, i0 = tree.knn(q, 32)
+        d1, i1 = back.knn(q, 32)
+        assert torch.equal(d0, d1) and torch.equal(i0, i1)
+    finally:
+        back.free()
+    # first node's child pointed past the node array
+    raw = bytearray(path.read_bytes())
+    hdr = 8 + 5 * 8 + 6 * 4 + 8 * 8
+    struct.pack_into("<i", raw, hdr + 8, 1 << 30)
+    bad.write_bytes(bytes(raw))
+    with pytest.raises(ValueError, match="child index"):
+        ckd.kdtree.load_tree(bad)
+    tree.free()
