@@ -62,6 +62,8 @@ def _load():
     lib.ckd_ball_fill.argtypes = [vp, vp, i64, dbl, i32, vp, vp, vp, vp]
     lib.ckd_box_count.argtypes = [vp, vp, i64, vp, C.POINTER(i64), vp]
     lib.ckd_box_fill.argtypes = [vp, vp, i64, vp, vp, vp]
+    lib.ckd_box_count_periodic.argtypes = [vp, vp, i64, vp, C.POINTER(i64), vp]
+    lib.ckd_box_fill_periodic.argtypes = [vp, vp, i64, vp, vp, vp]
     lib.ckd_tree_dump.argtypes = [vp] * 11
     lib.ckd_release_memory.argtypes = []
     lib.ckd_last_error.restype = C.c_char_p

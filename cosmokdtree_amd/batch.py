@@ -34,9 +34,10 @@ for _n in ("ckd_batch_plan", "ckd_batch_run", "ckd_batch_status", "ckd_batch_fre
     getattr(lib, _n).restype = C.c_int
 
 
-def build_forest(sets, leaf: int = 16, boxsize=None, dtype=None) -> KDTree:
+def build_forest(sets, leaf: int = 16, boxsize=None, dtype=None, idx_bits: int = 0) -> KDTree:
     """One tree per point set, built in one pass.  `sets` is a list of (n_i, 3)
-    arrays/tensors, or a pair (points (N, 3), offsets (nsets+1))."""
+    arrays/tensors, or a pair (points (N, 3), offsets (nsets+1)).  idx_bits
+    32 or 64 fixes the stored index width (0: 32 below 2^31 points)."""
     _require_gpu()
     if isinstance(sets, tuple) and len(sets) == 2:
         pts, off = sets
@@ -53,8 +54,8 @@ def build_forest(sets, leaf: int = 16, boxsize=None, dtype=None) -> KDTree:
     bs = None if boxsize is None else (C.c_double * d)(*[float(x) for x in boxsize])
     out = C.c_void_p()
     prec = 4 if dt == torch.float32 else 8
-    check(lib.ckd_build_forest(_vp(pts), off.ctypes.data_as(C.c_void_p), len(off) - 1, d, prec, int(leaf), bs, 0,
-                               _stream(), C.byref(out)), "build_forest")
+    check(lib.ckd_build_forest(_vp(pts), off.ctypes.data_as(C.c_void_p), len(off) - 1, d, prec, int(leaf), bs,
+                               int(idx_bits), _stream(), C.byref(out)), "build_forest")
     tree = KDTree(out.value, n=n, ndim=d, leafsize=int(leaf), torch_dtype=dt,
                   np_dtype=np.float32 if prec == 4 else np.float64, device=pts.device,
                   periodic=boxsize is not None,

@@ -17,7 +17,7 @@ int query_sph(const ckd_tree *t, const void *q, int64_t nq, int k, const void *m
 int query_ball(const ckd_tree *t, const void *q, int64_t nq, double r, int64_t *counts, const int64_t *off,
                void *dist, int64_t *idx, hipStream_t s);
 int query_box(const ckd_tree *t, const void *b, int64_t nb, int64_t *counts, const int64_t *off, int64_t *idx,
-              hipStream_t s);
+              int wrap, hipStream_t s);
 
 static thread_local char g_err[512] = "";
 
@@ -407,30 +407,58 @@ int ckd_ball_fill(const ckd_tree *t, const void *queries, int64_t nq, double rad
                         : segmented_sort<double>((double *)dist, idx, offsets, nq, 0, s);
 }
 
-int ckd_box_count(const ckd_tree *t, const void *boxes, int64_t nb, int64_t *offsets, int64_t *total,
-                  void *stream) {
+static int box_count(const ckd_tree *t, const void *boxes, int64_t nb, int64_t *offsets, int64_t *total, int wrap,
+                     void *stream) {
     g_err[0] = 0;
     if (!t || !offsets || nb < 0 || (nb > 0 && !boxes)) {
         set_error("invalid box_count arguments");
         return CKD_ERR_ARG;
     }
+    if (wrap && !t->periodic) {
+        set_error("boxes taken modulo the box need a periodic tree");
+        return CKD_ERR_ARG;
+    }
     hipStream_t s = (hipStream_t)stream;
     if (nb > 0) {
-        int rc = query_box(t, boxes, nb, offsets + 1, nullptr, nullptr, s);
+        int rc = query_box(t, boxes, nb, offsets + 1, nullptr, nullptr, wrap, s);
         if (rc) return rc;
     }
     return counts_to_offsets(offsets, nb, total, s);
 }
 
-int ckd_box_fill(const ckd_tree *t, const void *boxes, int64_t nb, const int64_t *offsets, int64_t *idx,
-                 void *stream) {
+static int box_fill(const ckd_tree *t, const void *boxes, int64_t nb, const int64_t *offsets, int64_t *idx, int wrap,
+                    void *stream) {
     g_err[0] = 0;
     if (!t || !offsets) {
         set_error("invalid box_fill arguments");
         return CKD_ERR_ARG;
     }
+    if (wrap && !t->periodic) {
+        set_error("boxes taken modulo the box need a periodic tree");
+        return CKD_ERR_ARG;
+    }
     if (nb <= 0) return CKD_OK;
-    return query_box(t, boxes, nb, nullptr, offsets, idx, (hipStream_t)stream);
+    return query_box(t, boxes, nb, nullptr, offsets, idx, wrap, (hipStream_t)stream);
+}
+
+int ckd_box_count(const ckd_tree *t, const void *boxes, int64_t nb, int64_t *offsets, int64_t *total,
+                  void *stream) {
+    return box_count(t, boxes, nb, offsets, total, 0, stream);
+}
+
+int ckd_box_fill(const ckd_tree *t, const void *boxes, int64_t nb, const int64_t *offsets, int64_t *idx,
+                 void *stream) {
+    return box_fill(t, boxes, nb, offsets, idx, 0, stream);
+}
+
+int ckd_box_count_periodic(const ckd_tree *t, const void *boxes, int64_t nb, int64_t *offsets, int64_t *total,
+                           void *stream) {
+    return box_count(t, boxes, nb, offsets, total, 1, stream);
+}
+
+int ckd_box_fill_periodic(const ckd_tree *t, const void *boxes, int64_t nb, const int64_t *offsets, int64_t *idx,
+                          void *stream) {
+    return box_fill(t, boxes, nb, offsets, idx, 1, stream);
 }
 
 int ckd_tree_dump(const ckd_tree *t, int32_t *axis, void *split, void *lo, void *hi, int32_t *child, int32_t *parent,

@@ -15,10 +15,10 @@ int query_ball_f32(const ckd_tree *, const void *, int64_t, double, int64_t *, c
                    hipStream_t);
 int query_ball_f64(const ckd_tree *, const void *, int64_t, double, int64_t *, const int64_t *, void *, int64_t *,
                    hipStream_t);
-int query_box_f32(const ckd_tree *, const void *, int64_t, int64_t *, const int64_t *, int64_t *, hipStream_t);
+int query_box_f32(const ckd_tree *, const void *, int64_t, int64_t *, const int64_t *, int64_t *, int, hipStream_t);
 int query_sph_f32(const ckd_tree *, const void *, int64_t, int, const void *, void *, void *, hipStream_t);
 int query_sph_f64(const ckd_tree *, const void *, int64_t, int, const void *, void *, void *, hipStream_t);
-int query_box_f64(const ckd_tree *, const void *, int64_t, int64_t *, const int64_t *, int64_t *, hipStream_t);
+int query_box_f64(const ckd_tree *, const void *, int64_t, int64_t *, const int64_t *, int64_t *, int, hipStream_t);
 
 int query_knn(const ckd_tree *t, const void *q, int64_t nq, int k, int sorted, const float *seed, void *dist,
               int64_t *idx, int64_t *stats, hipStream_t s) {
@@ -45,8 +45,9 @@ int query_sph(const ckd_tree *t, const void *q, int64_t nq, int k, const void *m
 }
 
 int query_box(const ckd_tree *t, const void *b, int64_t nb, int64_t *counts, const int64_t *off, int64_t *idx,
-              hipStream_t s) {
-    return t->prec == 4 ? query_box_f32(t, b, nb, counts, off, idx, s) : query_box_f64(t, b, nb, counts, off, idx, s);
+              int wrap, hipStream_t s) {
+    return t->prec == 4 ? query_box_f32(t, b, nb, counts, off, idx, wrap, s)
+                        : query_box_f64(t, b, nb, counts, off, idx, wrap, s);
 }
 
 }  // namespace ckd

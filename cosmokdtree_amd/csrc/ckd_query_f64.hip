@@ -36,8 +36,8 @@ int query_sph_f64(const ckd_tree *t, const void *q, int64_t nq, int k, const voi
 }
 
 int query_box_f64(const ckd_tree *t, const void *b, int64_t nb, int64_t *counts, const int64_t *off, int64_t *idx,
-                  hipStream_t s) {
-    return box_t<double>(t, b, nb, counts, off, idx, s);
+                  int wrap, hipStream_t s) {
+    return box_t<double>(t, b, nb, counts, off, idx, wrap, s);
 }
 
 }  // namespace ckd

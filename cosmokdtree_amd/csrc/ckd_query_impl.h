@@ -388,6 +388,39 @@ __global__ __launch_bounds__(QT) void k_box(TreeView<T> tv, const T *boxes, int6
     if (!FILL) counts[bi] = cnt;
 }
 
+// Boxes taken modulo a periodic tree's box (wrap_pieces): every sub-box of
+// the product of the axes' pieces (axis 0 fastest) is searched with the
+// reference's rules (box_one), its rows written after the previous ones.
+template <typename T, int D, bool FILL>
+__global__ __launch_bounds__(QT) void k_box_wrap(TreeView<T> tv, const T *boxes, int64_t nb, int64_t *counts,
+                                                 const int64_t *offsets, int64_t *iout) {
+    const int64_t bi = (int64_t)blockIdx.x * QT + threadIdx.x;
+    if (bi >= nb) return;
+    T pa[D][3], pb[D][3];
+    int np[D];
+    int total = 1;
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        np[d] = wrap_pieces<T>(boxes[bi * 2 * D + 2 * d], boxes[bi * 2 * D + 2 * d + 1], tv.L[d], pa[d], pb[d]);
+        total *= np[d];
+    }
+    int64_t cnt = 0;
+    int64_t *out = FILL ? iout + offsets[bi] : nullptr;
+    for (int c = 0; c < total; c++) {
+        T sb[2 * D];
+        int r = c;
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            const int j = r % np[d];
+            r /= np[d];
+            sb[2 * d] = j == 0 ? pa[d][0] : (j == 1 ? pa[d][1] : pa[d][2]);
+            sb[2 * d + 1] = j == 0 ? pb[d][0] : (j == 1 ? pb[d][1] : pb[d][2]);
+        }
+        cnt += box_one<T, D, FILL>(tv, sb, 0, FILL ? out + cnt : nullptr, 0);
+    }
+    if (!FILL) counts[bi] = cnt;
+}
+
 // ------------------------------------------------------------ dispatch
 
 // kNN kernel: CKD_KNN_LANE (one traversal per query, k_knn),
@@ -742,10 +775,13 @@ int ball_launch(const ckd_tree *t, const T *q, int64_t nq, T r2, int64_t *counts
 
 template <typename T, int D>
 int box_launch(const ckd_tree *t, const T *b, int64_t nb, int64_t *counts, const int64_t *off, int64_t *idx,
-               hipStream_t s) {
+               int wrap, hipStream_t s) {
     const TreeView<T> tv = view<T>(t);
     const unsigned g = (unsigned)ceil_div(nb, QT);
-    if (off) k_box<T, D, true><<<g, QT, 0, s>>>(tv, b, nb, counts, off, idx);
+    if (wrap) {
+        if (off) k_box_wrap<T, D, true><<<g, QT, 0, s>>>(tv, b, nb, counts, off, idx);
+        else k_box_wrap<T, D, false><<<g, QT, 0, s>>>(tv, b, nb, counts, off, idx);
+    } else if (off) k_box<T, D, true><<<g, QT, 0, s>>>(tv, b, nb, counts, off, idx);
     else k_box<T, D, false><<<g, QT, 0, s>>>(tv, b, nb, counts, off, idx);
     CKD_HIP(hipGetLastError());
     return CKD_OK;
@@ -785,9 +821,9 @@ int ball_t(const ckd_tree *t, const void *q, int64_t nq, double r, int64_t *coun
 }
 
 template <typename T>
-int box_t(const ckd_tree *t, const void *b, int64_t nb, int64_t *counts, const int64_t *off, int64_t *idx,
+int box_t(const ckd_tree *t, const void *b, int64_t nb, int64_t *counts, const int64_t *off, int64_t *idx, int wrap,
           hipStream_t s) {
-    CKD_DIM_SWITCH(t->ndim, (box_launch<T, D_>(t, (const T *)b, nb, counts, off, idx, s)))
+    CKD_DIM_SWITCH(t->ndim, (box_launch<T, D_>(t, (const T *)b, nb, counts, off, idx, wrap, s)))
     return CKD_ERR_ARG;
 }
 

@@ -1094,6 +1094,44 @@ __device__ __forceinline__ int64_t ball_stack(const TreeView<T> &tv, int2 root, 
     return cnt;
 }
 
+// Pieces of the interval [lo, hi] of one axis of a box taken modulo the
+// periodic box: [lo - kL, hi - kL] clipped to [-L/2, L/2] for k = m+1, m,
+// m-1 (ascending positions), m = floor((lo + L/2) / L) in double; pieces
+// that touch are merged, so they are disjoint.  The whole axis when
+// hi - lo >= L, no piece when lo > hi.  No reference counterpart:
+// box_search does not wrap (cosmokdtree.f90:1260); this is the library's
+// rule for boxes that straddle the faces of a periodic tree.
+template <typename T>
+__device__ __forceinline__ int wrap_pieces(T lo, T hi, T L, T (&pa)[3], T (&pb)[3]) {
+    const T H = L * T(0.5);
+    if (!(lo <= hi)) return 0;
+    if (hi - lo >= L) {
+        pa[0] = -H;
+        pb[0] = H;
+        return 1;
+    }
+    const double m = floor(((double)lo + (double)H) / (double)L);
+    int n = 0;
+#pragma unroll
+    for (int j = 1; j >= -1; j--) {
+        const T sh = (T)(m + j) * L;
+        T a = lo - sh, b = hi - sh;
+        a = a < -H ? -H : a;
+        b = b > H ? H : b;
+        if (a <= b) {
+            if (n > 0 && a <= pb[n - 1]) {
+                pb[n - 1] = b > pb[n - 1] ? b : pb[n - 1];
+            } else {
+                pa[n] = a;
+                pb[n] = b;
+                n++;
+            }
+        }
+    }
+    return n;
+}
+
+
 // Box search below `root` (box_search_recursive, cosmokdtree.f90:1258-1328):
 // visit the left child iff box_lo[axis] < split and the right child iff
 // box_hi[axis] > split; leaves test box_lo <= x <= box_hi on every axis.  A

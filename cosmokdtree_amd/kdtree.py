@@ -180,8 +180,12 @@ class KDTree(int):
                                 _vp(idx), _stream()), "ball_search")
         return off, dist[:total.value], idx[:total.value]
 
-    def box(self, boxes):
-        """Box search of device boxes (nb, 2*ndim): CSR (offsets, idx) on the GPU."""
+    def box(self, boxes, wrap: bool = False):
+        """Box search of device boxes (nb, 2*ndim): CSR (offsets, idx) on the GPU.
+
+        wrap=True (periodic trees) takes each box modulo the periodic box, so
+        boxes may straddle the faces (ckd_box_count_periodic); the default
+        follows the reference, which does not wrap (cosmokdtree.f90:1260)."""
         self._check_alive()
         b = torch.as_tensor(boxes, dtype=self.torch_dtype, device=self.device)
         if b.dim() == 1:
@@ -192,9 +196,11 @@ class KDTree(int):
         nb = b.shape[0]
         off = torch.empty(nb + 1, dtype=torch.int64, device=self.device)
         total = C.c_int64(0)
-        check(lib.ckd_box_count(self._ptr, _vp(b), nb, _vp(off), C.byref(total), _stream()), "box_search")
+        count, fill = ((lib.ckd_box_count_periodic, lib.ckd_box_fill_periodic) if wrap
+                       else (lib.ckd_box_count, lib.ckd_box_fill))
+        check(count(self._ptr, _vp(b), nb, _vp(off), C.byref(total), _stream()), "box_search")
         idx = torch.empty(max(total.value, 1), dtype=torch.int64, device=self.device)
-        check(lib.ckd_box_fill(self._ptr, _vp(b), nb, _vp(off), _vp(idx), _stream()), "box_search")
+        check(fill(self._ptr, _vp(b), nb, _vp(off), _vp(idx), _stream()), "box_search")
         return off, idx[:total.value]
 
     def dump(self) -> dict:
@@ -360,10 +366,11 @@ def box_search(tree, box):
     return idx.cpu().numpy()
 
 
-def box_search_batch(tree, boxes):
+def box_search_batch(tree, boxes, wrap=False):
     """Many boxes in one call (no reference counterpart: the reference takes
-    one box per call).  Returns CSR (offsets (nb+1), idx) as numpy arrays."""
+    one box per call).  Returns CSR (offsets (nb+1), idx) as numpy arrays.
+    wrap=True takes the boxes modulo a periodic tree's box (KDTree.box)."""
     t = _tree(tree)
-    boxes = np.asarray(boxes, dtype=t.np_dtype).reshape(-1, 2 * t.ndim)
-    off, idx = t.box(torch.from_numpy(boxes))
+    boxes = np.asarray(np.asarray(boxes, dtype=np.float64), dtype=t.np_dtype).reshape(-1, 2 * t.ndim)
+    off, idx = t.box(torch.from_numpy(boxes), wrap=bool(wrap))
     return off.cpu().numpy(), idx.cpu().numpy()

@@ -194,7 +194,7 @@ int ckd_ball_fill(const ckd_tree *tree, const void *queries, int64_t nq, double 
  * (cosmokdtree2py.f90:252-287) -> box_search (cosmokdtree.f90:1220-1253),
  * with the reference's inclusive point test and strict child-visit tests
  * (cosmokdtree.f90:1296-1302,1309-1325).  No periodic wrap, as in the
- * reference (cosmokdtree.f90:1260).
+ * reference (cosmokdtree.f90:1260); ckd_box_count_periodic wraps.
  */
 int ckd_box_count(const ckd_tree *tree, const void *boxes, int64_t nb, int64_t *offsets, int64_t *total,
                   void *stream);
@@ -203,6 +203,25 @@ int ckd_box_count(const ckd_tree *tree, const void *boxes, int64_t nb, int64_t *
    (cosmokdtree2py.f90:289-315). */
 int ckd_box_fill(const ckd_tree *tree, const void *boxes, int64_t nb, const int64_t *offsets, int64_t *idx,
                  void *stream);
+
+/*
+ * Box search in a periodic tree with boxes taken modulo the box: a box may
+ * straddle the faces or lie outside [-L/2, L/2].  No reference counterpart
+ * (box_search does not wrap, cosmokdtree.f90:1260); the same two passes and
+ * arguments as ckd_box_count / ckd_box_fill.  The rule: on each axis the
+ * interval [lo, hi] (in prec) becomes the pieces [lo - kL, hi - kL] clipped
+ * to [-L/2, L/2] for k = m+1, m, m-1, m = floor((lo + L/2) / L) evaluated
+ * in double, with pieces that touch merged; the whole axis when
+ * hi - lo >= L; nothing when lo > hi.  Every sub-box of the product of the
+ * pieces (axis 0 varying fastest) is searched with the reference's rules
+ * (inclusive point test, strict child visits, cosmokdtree.f90:1296-1325)
+ * and its rows follow the previous sub-box's.  The pieces are disjoint, so
+ * a row holds each point at most once.  CKD_ERR_ARG for a non-periodic tree.
+ */
+int ckd_box_count_periodic(const ckd_tree *tree, const void *boxes, int64_t nb, int64_t *offsets, int64_t *total,
+                           void *stream);
+int ckd_box_fill_periodic(const ckd_tree *tree, const void *boxes, int64_t nb, const int64_t *offsets, int64_t *idx,
+                          void *stream);
 
 /*
  * Copy the tree structure to host arrays (any pointer may be NULL), in the

@@ -115,6 +115,15 @@ void *ckdo_box_run(void *hv, const void *boxes, int64_t nb, int64_t *counts) {
     return rh;
 }
 
+void *ckdo_box_wrap_run(void *hv, const void *boxes, int64_t nb, int64_t *counts) {
+    ckdo_handle *h = (ckdo_handle *)hv;
+    ckdo_rhandle *rh = (ckdo_rhandle *)malloc(sizeof(ckdo_rhandle));
+    rh->prec = h->prec;
+    if (h->prec == 4) rh->r = box_wrap_run_f32((otree_f32 *)h->t, (const float *)boxes, nb, counts);
+    else rh->r = box_wrap_run_f64((otree_f64 *)h->t, (const double *)boxes, nb, counts);
+    return rh;
+}
+
 void ckdo_res_fetch(void *rv, const int64_t *offsets, void *dist, int64_t *idx) {
     ckdo_rhandle *rh = (ckdo_rhandle *)rv;
     if (rh->prec == 4) res_fetch_f32((ores_f32 *)rh->r, offsets, (float *)dist, idx);

@@ -694,6 +694,76 @@ static T(ores) *T(box_run)(const T(otree) *t, const REAL *boxes, int64_t nb, int
     return res;
 }
 
+/* Box search in a periodic tree with boxes that may straddle the faces (no
+   reference counterpart: box_search does not wrap, cosmokdtree.f90:1260).
+   The rule this library defines: on each axis the interval [lo, hi] is
+   taken modulo L as the pieces [lo - kL, hi - kL] clipped to [-L/2, L/2]
+   for k = m+1, m, m-1, m = floor((lo + L/2) / L) in double, pieces that
+   touch merged, the whole axis when hi - lo >= L, nothing when lo > hi.
+   Every sub-box of the product of the pieces (axis 0 fastest) is searched
+   with the reference's rules above, and the rows are concatenated: the
+   pieces are disjoint, so no point comes twice. */
+static int T(wrap_pieces)(REAL lo, REAL hi, REAL L, REAL *pa, REAL *pb) {
+    const REAL H = L * (REAL)0.5;
+    double m;
+    int j, n = 0;
+    if (!(lo <= hi)) return 0;
+    if (hi - lo >= L) {
+        pa[0] = -H;
+        pb[0] = H;
+        return 1;
+    }
+    m = floor(((double)lo + (double)H) / (double)L);
+    for (j = 1; j >= -1; j--) {
+        const REAL sh = (REAL)(m + j) * L;
+        REAL a = lo - sh, b = hi - sh;
+        if (a < -H) a = -H;
+        if (b > H) b = H;
+        if (a <= b) {
+            if (n > 0 && a <= pb[n - 1]) {
+                if (b > pb[n - 1]) pb[n - 1] = b;
+            } else {
+                pa[n] = a;
+                pb[n] = b;
+                n++;
+            }
+        }
+    }
+    return n;
+}
+
+static T(ores) *T(box_wrap_run)(const T(otree) *t, const REAL *boxes, int64_t nb, int64_t *counts) {
+    T(ores) *res = (T(ores) *)calloc(1, sizeof(T(ores)));
+    int64_t b;
+    res->nq = nb;
+    res->lists = (T(olist) *)calloc((size_t)(nb > 0 ? nb : 1), sizeof(T(olist)));
+#pragma omp parallel for schedule(dynamic, 16)
+    for (b = 0; b < nb; b++) {
+        T(olist) *l = &res->lists[b];
+        const REAL *bx = boxes + (size_t)b * 2 * t->ndim;
+        REAL pa[CKDO_MAXD][3], pb[CKDO_MAXD][3], sub[2 * CKDO_MAXD];
+        int np[CKDO_MAXD], d;
+        int64_t total = 1, c, j;
+        for (d = 0; d < t->ndim; d++) {
+            np[d] = T(wrap_pieces)(bx[2 * d], bx[2 * d + 1], t->L[d], pa[d], pb[d]);
+            total *= np[d];
+        }
+        for (c = 0; c < total; c++) {
+            int64_t r = c;
+            for (d = 0; d < t->ndim; d++) {
+                const int64_t jd = r % np[d];
+                r /= np[d];
+                sub[2 * d] = pa[d][jd];
+                sub[2 * d + 1] = pb[d][jd];
+            }
+            T(box_rec)(t, 0, sub, l);
+        }
+        for (j = 0; j < l->n; j++) l->i[j] -= 1;
+        counts[b] = l->n;
+    }
+    return res;
+}
+
 /* Preorder dump of the tree structure. */
 static void T(dump)(const T(otree) *t, int8_t *is_leaf, int32_t *axis, REAL *split, REAL *minb, REAL *maxb,
                     int64_t *left, int64_t *right, int64_t *start, int64_t *count, REAL *lpts, int64_t *lidx) {

@@ -63,6 +63,8 @@ def oracle_lib():
         lib.ckdo_ball_run.argtypes = [_vp, _vp, _i64, C.c_double, _i32, _vp]
         lib.ckdo_box_run.restype = _vp
         lib.ckdo_box_run.argtypes = [_vp, _vp, _i64, _vp]
+        lib.ckdo_box_wrap_run.restype = _vp
+        lib.ckdo_box_wrap_run.argtypes = [_vp, _vp, _i64, _vp]
         lib.ckdo_res_fetch.argtypes = [_vp, _vp, _vp, _vp]
         lib.ckdo_res_free.argtypes = [_vp]
         _oracle = lib
@@ -145,11 +147,16 @@ class OracleTree:
         off, d, i = self.ball_csr(q, r, sorted)
         return _split(off, d), _split(off, i)
 
-    def box_csr(self, boxes):
+    def box_csr(self, boxes, wrap=False):
+        """Box search; wrap=True takes boxes modulo the periodic box (the
+        straddling-box rule of ckdo_box_wrap_run)."""
         b = np.ascontiguousarray(np.asarray(boxes, dtype=self.dtype).reshape(-1, 2 * self.ndim))
         nb = b.shape[0]
         cnt = np.empty(nb, dtype=np.int64)
-        res = self.lib.ckdo_box_run(self.h, _ptr(b), nb, _ptr(cnt))
+        if wrap and not self.periodic:
+            raise ValueError("wrapped boxes need a periodic tree")
+        run = self.lib.ckdo_box_wrap_run if wrap else self.lib.ckdo_box_run
+        res = run(self.h, _ptr(b), nb, _ptr(cnt))
         off = _csr(cnt)
         i = np.empty(max(int(off[-1]), 1), dtype=np.int64)
         self.lib.ckdo_res_fetch(res, _ptr(off), None, _ptr(i))

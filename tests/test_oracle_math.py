@@ -183,3 +183,96 @@ def test_planted_mistakes_are_caught():
     with pytest.raises(AssertionError):
         assert_knn_rows(pts, q, 5, wd, wi, d, i, sorted_got=False)
     assert_knn_rows(pts, q, 5, bd, bi, d, i, sorted_got=False)
+
+
+# ---------------------------------------------------- periodic (wrapped) boxes
+
+def brute_wrap_box(pts, box, L):
+    """Indices of the points inside a box taken modulo the periodic box:
+    on every axis some whole number of periods j puts the coordinate in
+    [lo, hi] (inclusive), in float64."""
+    p = pts.astype(np.float64)
+    keep = np.ones(len(p), dtype=bool)
+    for d in range(p.shape[1]):
+        lo, hi, Ld = float(box[2 * d]), float(box[2 * d + 1]), float(L[d])
+        if hi - lo >= Ld:
+            continue
+        js = np.arange(np.floor((lo - 0.5 * Ld) / Ld) - 1, np.ceil((hi + 0.5 * Ld) / Ld) + 2)
+        x = p[:, d][:, None] + js[None, :] * Ld
+        keep &= np.any((x >= lo) & (x <= hi), axis=1)
+    return np.nonzero(keep)[0]
+
+
+def near_wrapped_edge(pts, box, L, tol):
+    """Points within tol of a box face, modulo L (where float32 rounding of
+    the shifted faces may decide differently from exact arithmetic)."""
+    p = pts.astype(np.float64)
+    near = np.zeros(len(p), dtype=bool)
+    for d in range(p.shape[1]):
+        for e in (float(box[2 * d]), float(box[2 * d + 1])):
+            r = np.mod(p[:, d] - e + 0.5 * L[d], L[d]) - 0.5 * L[d]
+            near |= np.abs(r) < tol
+    return near
+
+
+def straddling_boxes(rng, nb, L, dtype, D=3):
+    """Boxes of side log-uniform in [0.5, 5] (times L/500) centred anywhere in
+    the box, many crossing a face; a few wider than L on an axis, a few far
+    outside [-L/2, L/2] and a few with lo > hi."""
+    c = (rng.random((nb, D)) - 0.5) * L
+    c[: nb // 3] = np.sign(c[: nb // 3]) * (0.5 * L - rng.random((nb // 3, D)) * 0.01 * L)  # near the faces
+    side = np.exp(rng.uniform(np.log(0.5), np.log(5.0), (nb, D))) * L / 500.0 * 40
+    b = np.empty((nb, 2 * D))
+    b[:, 0::2] = c - side / 2
+    b[:, 1::2] = c + side / 2
+    b[-4, 0:2] = [-0.7 * L, 0.8 * L]  # wider than L on axis 0
+    b[-3, :] += 2 * L                  # two periods away
+    b[-2, :] -= 3 * L
+    b[-1, 2:4] = [1.0, -1.0]           # empty on axis 1
+    return b.astype(dtype)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_wrapped_box_matches_brute_force(dtype):
+    rng = np.random.default_rng(4242)
+    L = np.full(3, 100.0)
+    pts = ((rng.random((20000, 3)) - 0.5) * 100.0).astype(dtype)
+    pts[:200, 0] = -50.0  # points on the faces
+    pts[200:400, 2] = 50.0
+    t = OracleTree(pts, 16, L)
+    boxes = straddling_boxes(rng, 300, 100.0, dtype)
+    off, idx = t.box_csr(boxes, wrap=True)
+    crossing = 0
+    for b in range(len(boxes)):
+        got = idx[off[b]:off[b + 1]]
+        assert len(np.unique(got)) == len(got), f"box {b}: a point twice"
+        want = brute_wrap_box(pts, boxes[b], L)
+        if dtype == np.float64:
+            assert np.array_equal(np.sort(got), want), f"box {b}"
+        else:
+            diff = np.setxor1d(got, want)
+            assert np.all(near_wrapped_edge(pts[diff], boxes[b], L, 1e-4)), f"box {b}: {diff[:5]}"
+        crossing += bool(np.any(boxes[b, 0::2] < -50) or np.any(boxes[b, 1::2] > 50))
+    assert crossing > 50
+    # a box inside [-L/2, L/2] gives what the reference's box search gives
+    inner = np.array([[-10.0, 5.0, -3.0, 20.0, 0.0, 40.0]], dtype=dtype)
+    assert np.array_equal(t.box_csr(inner, wrap=True)[1], t.box_csr(inner)[1])
+
+
+def test_wrapped_box_planted_mistakes_are_caught():
+    """A rule that forgets the wrap on one axis, or shifts the wrong way,
+    disagrees with the brute-force membership."""
+    rng = np.random.default_rng(99)
+    L = np.full(3, 10.0)
+    pts = (rng.random((5000, 3)) - 0.5) * 10.0
+    box = np.array([4.0, 6.0, -1.0, 1.0, -6.0, -3.5])  # crosses +L/2 on x and -L/2 on z
+    want = brute_wrap_box(pts, box, L)
+    # no wrap on z: clip only
+    clipped = box.copy()
+    clipped[4] = -5.0
+    t = OracleTree(pts, 16, L)
+    off, got = t.box_csr(np.array([[4.0, 5.0, -1.0, 1.0, -5.0, -3.5]]))
+    assert not np.array_equal(np.sort(got), want)
+    # shifted the wrong way on x: [4 + 10, 6 + 10] misses everything
+    assert len(brute_wrap_box(pts, box, L)) > len(t.box_csr(np.array([[-5.0, -4.0, -1.0, 1.0, -5.0, -3.5]]))[1])
+    assert np.array_equal(np.sort(t.box_csr(box[None], wrap=True)[1]), want)
