@@ -8,14 +8,14 @@
 
 namespace ckd {
 int query_knn(const ckd_tree *t, const void *q, int64_t nq, int k, int sorted, const float *seed, void *dist,
-              int64_t *idx, int64_t *stats, hipStream_t s);
+              int64_t *idx, int64_t *stats, Scratch &sc, hipStream_t s);
 int query_knn_learned(const ckd_tree *t, const void *q, int64_t nq, int k, int sorted, const float *params,
                       float margin, int G, const uint16_t *table, void *dist, int64_t *idx, int64_t *stats,
-                      hipStream_t s);
+                      Scratch &sc, hipStream_t s);
 int query_sph(const ckd_tree *t, const void *q, int64_t nq, int k, const void *mass, void *h, void *rho,
-              hipStream_t s);
+              Scratch &sc, hipStream_t s);
 int query_ball(const ckd_tree *t, const void *q, int64_t nq, double r, int64_t *counts, const int64_t *off,
-               void *dist, int64_t *idx, hipStream_t s);
+               void *dist, int64_t *idx, Scratch &sc, hipStream_t s);
 int query_box(const ckd_tree *t, const void *b, int64_t nb, int64_t *counts, const int64_t *off, int64_t *idx,
               int wrap, hipStream_t s);
 
@@ -33,26 +33,21 @@ int hip_fail(hipError_t e, const char *what, const char *file, int line) {
     return e == hipErrorOutOfMemory ? CKD_ERR_MEMORY : CKD_ERR_HIP;
 }
 
-static int grow(void **buf, size_t *have, size_t bytes, void **p, hipStream_t s) {
-    if (*have < bytes) {
-        if (*buf) {
-            // the old buffer may still be read by a call on another stream
-            CKD_HIP(hipDeviceSynchronize());
-            dev_free(*buf, s);
+int Scratch::take(size_t bytes, void **p) {
+    *p = nullptr;
+    if (arena) {
+        const size_t at = (used + 255) & ~(size_t)255;
+        if (at + bytes > cap) {
+            set_error("the call needs %zu bytes of scratch; the static scratch holds %zu", at + bytes, cap);
+            return CKD_ERR_LIMIT;
         }
-        *buf = nullptr;
-        *have = 0;
-        const size_t want = bytes + bytes / 4;
-        int rc = dev_alloc(buf, want, s);
-        if (rc) return rc;
-        *have = want;
+        used = at + bytes;
+        *p = arena + at;
+        return CKD_OK;
     }
-    *p = *buf;
-    return CKD_OK;
-}
-
-int workspace(ckd_tree *t, int slot, size_t bytes, void **p, hipStream_t s) {
-    return grow(&t->ws[slot], &t->ws_bytes[slot], bytes, p, s);
+    const int rc = dev_alloc(p, bytes, s);
+    if (rc == CKD_OK) owned.push_back(*p);
+    return rc;
 }
 
 // Frees every device array of the tree.  Like hipFree, it first waits for
@@ -67,16 +62,11 @@ static void release(ckd_tree *t) {
     (void)hipGetDevice(&cur);
     if (cur != t->device) (void)hipSetDevice(t->device);
     (void)hipDeviceSynchronize();
-    void *arrays[] = {t->set_off, t->ws[0], t->ws[1], t->ws[2], t->nodes,  t->bbox,  t->dbox,
-                      t->range,   t->pts,   t->idx,   t->wide,  t->wroot, t->wrange, t->wparent};
-    static_assert(ckd_tree::NWS == 3, "workspace slots listed above");
+    void *arrays[] = {t->set_off, t->nodes, t->bbox, t->dbox,  t->range,  t->pts,
+                      t->idx,     t->wide,  t->wroot, t->wrange, t->wparent};
     for (void *a : arrays) dev_free_idle(a);
     if (cur != t->device) (void)hipSetDevice(cur);
     t->set_off = nullptr;
-    for (int i = 0; i < ckd_tree::NWS; i++) {
-        t->ws[i] = nullptr;
-        t->ws_bytes[i] = 0;
-    }
     t->nodes = t->bbox = t->dbox = t->pts = t->idx = t->wide = nullptr;
     t->wroot = t->wrange = nullptr;
     t->wparent = nullptr;
@@ -264,7 +254,8 @@ int ckd_knn(const ckd_tree *t, const void *queries, int64_t nq, int32_t k, int32
         return CKD_ERR_ARG;
     }
     if (nq == 0) return CKD_OK;
-    return query_knn(t, queries, nq, k, sorted, seed_r, dist, idx, stats, (hipStream_t)stream);
+    Scratch sc((hipStream_t)stream);
+    return query_knn(t, queries, nq, k, sorted, seed_r, dist, idx, stats, sc, (hipStream_t)stream);
 }
 
 int ckd_knn_learned(const ckd_tree *t, const void *queries, int64_t nq, int32_t k, int32_t sorted,
@@ -292,7 +283,8 @@ int ckd_knn_learned(const ckd_tree *t, const void *queries, int64_t nq, int32_t 
         return CKD_ERR_ARG;
     }
     if (nq == 0) return CKD_OK;
-    return query_knn_learned(t, queries, nq, k, sorted, params, margin, G, table, dist, idx, stats,
+    Scratch sc((hipStream_t)stream);
+    return query_knn_learned(t, queries, nq, k, sorted, params, margin, G, table, dist, idx, stats, sc,
                              (hipStream_t)stream);
 }
 
@@ -316,7 +308,8 @@ int ckd_sph_density(const ckd_tree *t, const void *queries, int64_t nq, int32_t 
         return CKD_ERR_ARG;
     }
     if (nq == 0) return CKD_OK;
-    return query_sph(t, queries, nq, k, mass, h, rho, (hipStream_t)stream);
+    Scratch sc((hipStream_t)stream);
+    return query_sph(t, queries, nq, k, mass, h, rho, sc, (hipStream_t)stream);
 }
 
 int ckd_knn_config(ckd_tree *t, int32_t kernel, int32_t stack_depth, int32_t queries_per_wave) {
@@ -352,6 +345,7 @@ int ckd_knn_timing(ckd_tree *t, int32_t on, double *total_ms, int64_t *launches)
     double sum = 0;
     int64_t cnt = 0;
     int rc = CKD_OK;
+    std::lock_guard<std::mutex> lock(t->timing_mu);
     for (auto &ev : t->knn_events) {
         float ms = 0;
         hipError_t e = hipEventSynchronize(ev.second);
@@ -386,7 +380,8 @@ int ckd_ball_count(const ckd_tree *t, const void *queries, int64_t nq, double ra
     }
     hipStream_t s = (hipStream_t)stream;
     if (nq > 0) {
-        int rc = query_ball(t, queries, nq, radius, offsets + 1, nullptr, nullptr, nullptr, s);
+        Scratch sc(s);
+        int rc = query_ball(t, queries, nq, radius, offsets + 1, nullptr, nullptr, nullptr, sc, s);
         if (rc) return rc;
     }
     return counts_to_offsets(offsets, nq, total, s);
@@ -401,7 +396,11 @@ int ckd_ball_fill(const ckd_tree *t, const void *queries, int64_t nq, double rad
     }
     if (nq <= 0) return CKD_OK;
     hipStream_t s = (hipStream_t)stream;
-    int rc = query_ball(t, queries, nq, radius, nullptr, offsets, dist, idx, s);
+    int rc;
+    {
+        Scratch sc(s);
+        rc = query_ball(t, queries, nq, radius, nullptr, offsets, dist, idx, sc, s);
+    }
     if (rc || !sorted) return rc;
     return t->prec == 4 ? segmented_sort<float>((float *)dist, idx, offsets, nq, 0, s)
                         : segmented_sort<double>((double *)dist, idx, offsets, nq, 0, s);

@@ -33,6 +33,7 @@
 #include <stdint.h>
 
 #include <cfloat>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -103,6 +104,27 @@ int hip_fail(hipError_t e, const char *what, const char *file, int line);
 int dev_alloc(void **p, size_t bytes, hipStream_t s);
 void dev_free(void *p, hipStream_t s);
 void dev_free_idle(void *p);
+
+// Device scratch of one library call, released when the call returns.  By
+// default each take() is a block of the caching allocator, handed back
+// stream-ordered at the end of the call, so calls that overlap on one tree
+// (other streams, other host threads) never share scratch.  Given an arena
+// (the entry points that allocate nothing), take() carves 256-byte aligned
+// pieces out of it instead and fails with CKD_ERR_LIMIT past its end.
+struct Scratch {
+    hipStream_t s;
+    char *arena = nullptr;
+    size_t cap = 0, used = 0;
+    std::vector<void *> owned;
+    explicit Scratch(hipStream_t s_) : s(s_) {}
+    Scratch(hipStream_t s_, void *a, size_t c) : s(s_), arena((char *)a), cap(c) {}
+    Scratch(const Scratch &) = delete;
+    Scratch &operator=(const Scratch &) = delete;
+    ~Scratch() {
+        for (void *b : owned) dev_free(b, s);
+    }
+    int take(size_t bytes, void **p);
+};
 
 // ---------------------------------------------------- device arithmetic
 // |x|, min, max as single instructions (source modifiers, v_min/v_max);
@@ -276,12 +298,8 @@ struct ckd_tree {
     int knn_stack = 0;       // stack depth of the stack kernel, 0 = automatic
     int knn_qpw = 0;         // queries per wave of the stack kernel (1..64), 0 = 64
     int timing = 0;          // record events around kNN traversal launches (ckd_knn_timing)
+    std::mutex timing_mu;    // guards knn_events: queries may run from several host threads
     std::vector<std::pair<hipEvent_t, hipEvent_t>> knn_events;
-    // grow-only query workspaces (calls on one tree must not overlap):
-    // slot 0 query ordering, 1 seed retry, 2 learned features and seeds
-    static constexpr int NWS = 3;
-    void *ws[NWS] = {nullptr, nullptr, nullptr};
-    size_t ws_bytes[NWS] = {0, 0, 0};
 };
 
 namespace ckd {
@@ -303,9 +321,6 @@ TreeView<T> view(const ckd_tree *t) {
     return v;
 }
 int build_tree(ckd_tree *t, const void *points, hipStream_t s);
-// Device workspace `slot` of at least `bytes` owned by the tree (grown on
-// demand; growing synchronises `s` before releasing the old buffer).
-int workspace(ckd_tree *t, int slot, size_t bytes, void **p, hipStream_t s);
 // wparent from the wide records (after a build or a load of a tree with them).
 int wide_parents(ckd_tree *t, hipStream_t s);
 }  // namespace ckd

@@ -5,8 +5,8 @@ namespace ckd {
 
 int query_knn_f32_open(const ckd_tree *t, const void *q, int64_t nq, int k, int sorted, const float *seed,
                       const float *params, float margin, int G, const uint16_t *table, void *dist, int64_t *idx,
-                      int64_t *stats, hipStream_t s) {
-    return knn_per<float, false>(t, q, nq, k, sorted, seed, params, margin, G, table, dist, idx, stats, s);
+                      int64_t *stats, Scratch &sc, hipStream_t s) {
+    return knn_per<float, false>(t, q, nq, k, sorted, seed, params, margin, G, table, dist, idx, stats, sc, s);
 }
 
 }  // namespace ckd

@@ -4,44 +4,45 @@
 namespace ckd {
 
 int query_knn_f32(const ckd_tree *, const void *, int64_t, int, int, const float *, void *, int64_t *, int64_t *,
-                  hipStream_t);
+                  Scratch &, hipStream_t);
 int query_knn_f64(const ckd_tree *, const void *, int64_t, int, int, const float *, void *, int64_t *, int64_t *,
-                  hipStream_t);
+                  Scratch &, hipStream_t);
 int query_knn_learned_f32(const ckd_tree *, const void *, int64_t, int, int, const float *, float, int,
-                          const uint16_t *, void *, int64_t *, int64_t *, hipStream_t);
+                          const uint16_t *, void *, int64_t *, int64_t *, Scratch &, hipStream_t);
 int query_knn_learned_f64(const ckd_tree *, const void *, int64_t, int, int, const float *, float, int,
-                          const uint16_t *, void *, int64_t *, int64_t *, hipStream_t);
+                          const uint16_t *, void *, int64_t *, int64_t *, Scratch &, hipStream_t);
 int query_ball_f32(const ckd_tree *, const void *, int64_t, double, int64_t *, const int64_t *, void *, int64_t *,
-                   hipStream_t);
+                   Scratch &, hipStream_t);
 int query_ball_f64(const ckd_tree *, const void *, int64_t, double, int64_t *, const int64_t *, void *, int64_t *,
-                   hipStream_t);
+                   Scratch &, hipStream_t);
 int query_box_f32(const ckd_tree *, const void *, int64_t, int64_t *, const int64_t *, int64_t *, int, hipStream_t);
-int query_sph_f32(const ckd_tree *, const void *, int64_t, int, const void *, void *, void *, hipStream_t);
-int query_sph_f64(const ckd_tree *, const void *, int64_t, int, const void *, void *, void *, hipStream_t);
+int query_sph_f32(const ckd_tree *, const void *, int64_t, int, const void *, void *, void *, Scratch &, hipStream_t);
+int query_sph_f64(const ckd_tree *, const void *, int64_t, int, const void *, void *, void *, Scratch &, hipStream_t);
 int query_box_f64(const ckd_tree *, const void *, int64_t, int64_t *, const int64_t *, int64_t *, int, hipStream_t);
 
 int query_knn(const ckd_tree *t, const void *q, int64_t nq, int k, int sorted, const float *seed, void *dist,
-              int64_t *idx, int64_t *stats, hipStream_t s) {
-    return t->prec == 4 ? query_knn_f32(t, q, nq, k, sorted, seed, dist, idx, stats, s)
-                        : query_knn_f64(t, q, nq, k, sorted, seed, dist, idx, stats, s);
+              int64_t *idx, int64_t *stats, Scratch &sc, hipStream_t s) {
+    return t->prec == 4 ? query_knn_f32(t, q, nq, k, sorted, seed, dist, idx, stats, sc, s)
+                        : query_knn_f64(t, q, nq, k, sorted, seed, dist, idx, stats, sc, s);
 }
 
 int query_knn_learned(const ckd_tree *t, const void *q, int64_t nq, int k, int sorted, const float *params,
                       float margin, int G, const uint16_t *table, void *dist, int64_t *idx, int64_t *stats,
-                      hipStream_t s) {
-    return t->prec == 4 ? query_knn_learned_f32(t, q, nq, k, sorted, params, margin, G, table, dist, idx, stats, s)
-                        : query_knn_learned_f64(t, q, nq, k, sorted, params, margin, G, table, dist, idx, stats, s);
+                      Scratch &sc, hipStream_t s) {
+    return t->prec == 4 ? query_knn_learned_f32(t, q, nq, k, sorted, params, margin, G, table, dist, idx, stats, sc, s)
+                        : query_knn_learned_f64(t, q, nq, k, sorted, params, margin, G, table, dist, idx, stats, sc, s);
 }
 
 int query_ball(const ckd_tree *t, const void *q, int64_t nq, double r, int64_t *counts, const int64_t *off,
-               void *dist, int64_t *idx, hipStream_t s) {
-    return t->prec == 4 ? query_ball_f32(t, q, nq, r, counts, off, dist, idx, s)
-                        : query_ball_f64(t, q, nq, r, counts, off, dist, idx, s);
+               void *dist, int64_t *idx, Scratch &sc, hipStream_t s) {
+    return t->prec == 4 ? query_ball_f32(t, q, nq, r, counts, off, dist, idx, sc, s)
+                        : query_ball_f64(t, q, nq, r, counts, off, dist, idx, sc, s);
 }
 
 int query_sph(const ckd_tree *t, const void *q, int64_t nq, int k, const void *mass, void *h, void *rho,
-              hipStream_t s) {
-    return t->prec == 4 ? query_sph_f32(t, q, nq, k, mass, h, rho, s) : query_sph_f64(t, q, nq, k, mass, h, rho, s);
+              Scratch &sc, hipStream_t s) {
+    return t->prec == 4 ? query_sph_f32(t, q, nq, k, mass, h, rho, sc, s)
+                        : query_sph_f64(t, q, nq, k, mass, h, rho, sc, s);
 }
 
 int query_box(const ckd_tree *t, const void *b, int64_t nb, int64_t *counts, const int64_t *off, int64_t *idx,

@@ -5,34 +5,34 @@
 namespace ckd {
 
 int query_knn_f32_per(const ckd_tree *, const void *, int64_t, int, int, const float *, const float *, float,
-                      int, const uint16_t *, void *, int64_t *, int64_t *, hipStream_t);
+                      int, const uint16_t *, void *, int64_t *, int64_t *, Scratch &, hipStream_t);
 int query_knn_f32_open(const ckd_tree *, const void *, int64_t, int, int, const float *, const float *, float,
-                       int, const uint16_t *, void *, int64_t *, int64_t *, hipStream_t);
+                       int, const uint16_t *, void *, int64_t *, int64_t *, Scratch &, hipStream_t);
 
 int query_knn_f32(const ckd_tree *t, const void *q, int64_t nq, int k, int sorted, const float *seed, void *dist,
-                  int64_t *idx, int64_t *stats, hipStream_t s) {
+                  int64_t *idx, int64_t *stats, Scratch &sc, hipStream_t s) {
     return t->periodic
-               ? query_knn_f32_per(t, q, nq, k, sorted, seed, nullptr, 0.0f, 0, nullptr, dist, idx, stats, s)
-               : query_knn_f32_open(t, q, nq, k, sorted, seed, nullptr, 0.0f, 0, nullptr, dist, idx, stats, s);
+               ? query_knn_f32_per(t, q, nq, k, sorted, seed, nullptr, 0.0f, 0, nullptr, dist, idx, stats, sc, s)
+               : query_knn_f32_open(t, q, nq, k, sorted, seed, nullptr, 0.0f, 0, nullptr, dist, idx, stats, sc, s);
 }
 
 int query_knn_learned_f32(const ckd_tree *t, const void *q, int64_t nq, int k, int sorted, const float *params,
                           float margin, int G, const uint16_t *table, void *dist, int64_t *idx, int64_t *stats,
-                          hipStream_t s) {
+                          Scratch &sc, hipStream_t s) {
     return t->periodic
-               ? query_knn_f32_per(t, q, nq, k, sorted, nullptr, params, margin, G, table, dist, idx, stats, s)
-               : query_knn_f32_open(t, q, nq, k, sorted, nullptr, params, margin, G, table, dist, idx, stats, s);
+               ? query_knn_f32_per(t, q, nq, k, sorted, nullptr, params, margin, G, table, dist, idx, stats, sc, s)
+               : query_knn_f32_open(t, q, nq, k, sorted, nullptr, params, margin, G, table, dist, idx, stats, sc, s);
 }
 
 int query_ball_f32(const ckd_tree *t, const void *q, int64_t nq, double r, int64_t *counts, const int64_t *off,
-                   void *dist, int64_t *idx, hipStream_t s) {
-    return ball_t<float>(t, q, nq, r, counts, off, dist, idx, s);
+                   void *dist, int64_t *idx, Scratch &sc, hipStream_t s) {
+    return ball_t<float>(t, q, nq, r, counts, off, dist, idx, sc, s);
 }
 
 int query_sph_f32(const ckd_tree *t, const void *q, int64_t nq, int k, const void *mass, void *h, void *rho,
-                   hipStream_t s) {
-    return t->periodic ? sph_launch<float, true>(t, (const float *)q, nq, k, (const float *)mass, (float *)h, (float *)rho, s)
-                       : sph_launch<float, false>(t, (const float *)q, nq, k, (const float *)mass, (float *)h, (float *)rho, s);
+                   Scratch &sc, hipStream_t s) {
+    return t->periodic ? sph_launch<float, true>(t, (const float *)q, nq, k, (const float *)mass, (float *)h, (float *)rho, sc, s)
+                       : sph_launch<float, false>(t, (const float *)q, nq, k, (const float *)mass, (float *)h, (float *)rho, sc, s);
 }
 
 int query_box_f32(const ckd_tree *t, const void *b, int64_t nb, int64_t *counts, const int64_t *off, int64_t *idx,

@@ -14,6 +14,23 @@ namespace {
 constexpr int QT = 256;  // threads per block
 constexpr int64_t ORDER_MIN_QUERIES = 4096;  // below this, queries run in input order
 
+// Scratch bytes of a kNN call: the leaf ordering of the queries (leaf_of,
+// rank, order as int32 x nq; leaf counts and offsets as int64 x nnodes+1;
+// scan scratch), taken when the batch is large enough to be ordered, and
+// the re-search lists of a seeded call (flag and offsets int64 x nq+1,
+// two lists int32 x nq, next seeds float x nq, a counter, scan scratch).
+inline size_t order_scratch_bytes(const ckd_tree *t, int64_t nq) {
+    const size_t a = 3 * sizeof(int32_t) * (size_t)nq, b = sizeof(int64_t) * (size_t)(t->nnodes + 1);
+    return a + 2 * b + scan_tmp_bytes(t->nnodes) + 256;
+}
+inline bool orders_queries(const ckd_tree *t, int64_t nq) {
+    return nq >= ORDER_MIN_QUERIES && nq < INT32_MAX && t->nnodes > 1;
+}
+inline size_t retry_scratch_bytes(int64_t nq) {
+    const size_t nb = (size_t)nq;
+    return 2 * 8 * (nb + 1) + 2 * 4 * nb + 4 * nb + 64 + scan_tmp_bytes(nq) + 8 * 256;  // 7 aligned takes
+}
+
 // Query ordering: each query's leaf, and its rank among the queries of
 // that leaf; a counting sort then lists the queries leaf by leaf so a wave
 // works on neighbouring queries.
@@ -481,18 +498,21 @@ int launch_wave(const ckd_tree *t, const TreeView<T> &tv, int kind, int sd, int 
 // Events around the main traversal launch, when the tree records them.
 struct LaunchTimer {
     const ckd_tree *t;
+    bool allowed = true;  // false for the entry points that create nothing
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int start(hipStream_t s) {
-        if (!t->timing) return CKD_OK;
+        if (!t->timing || !allowed) return CKD_OK;
         CKD_HIP(hipEventCreate(&e0));
         CKD_HIP(hipEventCreate(&e1));
         CKD_HIP(hipEventRecord(e0, s));
         return CKD_OK;
     }
     int stop(hipStream_t s) {
-        if (!t->timing) return CKD_OK;
+        if (!t->timing || !allowed) return CKD_OK;
         CKD_HIP(hipEventRecord(e1, s));
-        const_cast<ckd_tree *>(t)->knn_events.emplace_back(e0, e1);
+        ckd_tree *tw = const_cast<ckd_tree *>(t);  // the event list only; the tree itself stays read-only
+        std::lock_guard<std::mutex> lock(tw->timing_mu);
+        tw->knn_events.emplace_back(e0, e1);
         return CKD_OK;
     }
 };
@@ -500,9 +520,9 @@ struct LaunchTimer {
 template <typename T, int D, bool PER, int KMAX>
 int knn_launch_k(const ckd_tree *t, const TreeView<T> &tv, int kind, int sd, int qpw, const T *q, int64_t nq, int k,
                  int sorted, const float *seed, const int32_t *order, const int32_t *leaf_of, T *dist, int64_t *idx,
-                 int64_t *stats, hipStream_t s) {
+                 int64_t *stats, Scratch &sc, hipStream_t s) {
     const unsigned g = (unsigned)ceil_div(nq, QT);
-    LaunchTimer tm{t};
+    LaunchTimer tm{t, sc.arena == nullptr};
     int rc;
     if (kind == CKD_KNN_LANE) {
         if ((rc = tm.start(s))) return rc;
@@ -523,9 +543,8 @@ int knn_launch_k(const ckd_tree *t, const TreeView<T> &tv, int kind, int sd, int
     // workspace: flag, off (int64 x nq+1), list, list2 (int32 x nq),
     // seed_next (float x nq), count2, scan scratch
     const size_t nb = (size_t)nq;
-    const size_t bytes = 2 * 8 * (nb + 1) + 2 * 4 * nb + 4 * nb + 64 + scan_tmp_bytes(nq) + 8 * 256;  // 7 takes
     void *ws = nullptr;
-    if ((rc = workspace(const_cast<ckd_tree *>(t), 1, bytes, &ws, s))) return rc;
+    if ((rc = sc.take(retry_scratch_bytes(nq), &ws))) return rc;
     char *p = (char *)ws;
     auto take = [&](size_t b) {
         char *r = p;
@@ -560,10 +579,10 @@ int knn_launch_k(const ckd_tree *t, const TreeView<T> &tv, int kind, int sd, int
     return CKD_OK;
 }
 
-// Workspace slot 2: the features (nq x 32 bf16) and seeds (nq floats).
-inline int learned_buffers(const ckd_tree *t, int64_t nq, uint16_t **X, float **seeds, hipStream_t s) {
+// Scratch of the learned seeds: the features (nq x 32 bf16) and seeds (nq floats).
+inline int learned_buffers(Scratch &sc, int64_t nq, uint16_t **X, float **seeds) {
     void *ws = nullptr;
-    const int rc = workspace(const_cast<ckd_tree *>(t), 2, (size_t)nq * (64 + 4) + 256, &ws, s);
+    const int rc = sc.take((size_t)nq * (64 + 4) + 256, &ws);
     if (rc) return rc;
     *X = (uint16_t *)ws;
     *seeds = (float *)((char *)ws + (size_t)nq * 64);
@@ -572,7 +591,7 @@ inline int learned_buffers(const ckd_tree *t, int64_t nq, uint16_t **X, float **
 
 template <typename T, int D, bool PER>
 int knn_launch(const ckd_tree *t, const T *q, int64_t nq, int k, int sorted, const float *seed, const Learned *ln,
-               T *dist, int64_t *idx, int64_t *stats, hipStream_t s) {
+               T *dist, int64_t *idx, int64_t *stats, Scratch &sc, hipStream_t s) {
     const TreeView<T> tv = view<T>(t);
     const unsigned g = (unsigned)ceil_div(nq, QT);
     if (k <= 64 && t->n < INT32_MAX) {
@@ -602,12 +621,12 @@ int knn_launch(const ckd_tree *t, const T *q, int64_t nq, int k, int sorted, con
         uint16_t *X = nullptr;
         float *seeds = nullptr;
         bool feats_done = false;
-        if (nq >= ORDER_MIN_QUERIES && nq < INT32_MAX && t->nnodes > 1) {
-            // workspace: leaf_of, rank, order (int32 x nq), leaf counts and
+        if (orders_queries(t, nq)) {
+            // scratch: leaf_of, rank, order (int32 x nq), leaf counts and
             // offsets (int64 x nnodes+1), scan scratch
             const size_t a = 3 * sizeof(int32_t) * (size_t)nq, b = sizeof(int64_t) * (size_t)(t->nnodes + 1);
             void *ws = nullptr;
-            int rc = workspace(const_cast<ckd_tree *>(t), 0, a + 2 * b + scan_tmp_bytes(t->nnodes) + 256, &ws, s);
+            int rc = sc.take(order_scratch_bytes(t, nq), &ws);
             if (rc) return rc;
             int32_t *lo = (int32_t *)ws, *rk = lo + nq, *ord = rk + nq;
             int64_t *cnt = (int64_t *)((char *)ws + ((a + 255) & ~(size_t)255));
@@ -616,7 +635,7 @@ int knn_launch(const ckd_tree *t, const T *q, int64_t nq, int k, int sorted, con
             k_fill_i64<<<(unsigned)ceil_div(t->nnodes, 256), 256, 0, s>>>(cnt, t->nnodes, 0);
             if constexpr (D == 3) {
                 if (ln) {  // learned: the features come with the leaves
-                    if ((rc = learned_buffers(t, nq, &X, &seeds, s))) return rc;
+                    if ((rc = learned_buffers(sc, nq, &X, &seeds))) return rc;
                     k_locate_feat<T><<<g, QT, 0, s>>>(tv, q, nq, lo, rk, cnt, t->L[0], t->L[1], t->L[2], ln->G,
                                                       ln->table, X);
                     feats_done = true;
@@ -634,16 +653,16 @@ int knn_launch(const ckd_tree *t, const T *q, int64_t nq, int k, int sorted, con
             // them), then the network
             int rc;
             if (!feats_done) {
-                if ((rc = learned_buffers(t, nq, &X, &seeds, s))) return rc;
+                if ((rc = learned_buffers(sc, nq, &X, &seeds))) return rc;
                 if ((rc = ckd_mlp_features(t, q, nq, ln->G, ln->table, leaf_of, X, s))) return rc;
             }
             if ((rc = ckd_mlp_forward(ln->params, X, nq, ln->margin, 1, seeds, s))) return rc;
             seed = seeds;
         }
-        if (k <= 8) return knn_launch_k<T, D, PER, 8>(t, tv, kind, sd, qpw, q, nq, k, sorted, seed, order, leaf_of, dist, idx, stats, s);
-        if (k <= 16) return knn_launch_k<T, D, PER, 16>(t, tv, kind, sd, qpw, q, nq, k, sorted, seed, order, leaf_of, dist, idx, stats, s);
-        if (k <= 32) return knn_launch_k<T, D, PER, 32>(t, tv, kind, sd, qpw, q, nq, k, sorted, seed, order, leaf_of, dist, idx, stats, s);
-        return knn_launch_k<T, D, PER, 64>(t, tv, kind, sd, qpw, q, nq, k, sorted, seed, order, leaf_of, dist, idx, stats, s);
+        if (k <= 8) return knn_launch_k<T, D, PER, 8>(t, tv, kind, sd, qpw, q, nq, k, sorted, seed, order, leaf_of, dist, idx, stats, sc, s);
+        if (k <= 16) return knn_launch_k<T, D, PER, 16>(t, tv, kind, sd, qpw, q, nq, k, sorted, seed, order, leaf_of, dist, idx, stats, sc, s);
+        if (k <= 32) return knn_launch_k<T, D, PER, 32>(t, tv, kind, sd, qpw, q, nq, k, sorted, seed, order, leaf_of, dist, idx, stats, sc, s);
+        return knn_launch_k<T, D, PER, 64>(t, tv, kind, sd, qpw, q, nq, k, sorted, seed, order, leaf_of, dist, idx, stats, sc, s);
     }
     if (ln) {
         set_error("learned seeds need k <= 64");
@@ -658,11 +677,11 @@ int knn_launch(const ckd_tree *t, const T *q, int64_t nq, int k, int sorted, con
 // Leaf order of the queries (k_locate + counting sort) into workspace slot 0.
 template <typename T, int D>
 int order_queries(const ckd_tree *t, const TreeView<T> &tv, const T *q, int64_t nq, const int32_t **order,
-                  const int32_t **leaf_of, hipStream_t s) {
+                  const int32_t **leaf_of, Scratch &sc, hipStream_t s) {
     const unsigned g = (unsigned)ceil_div(nq, QT);
     const size_t a = 3 * sizeof(int32_t) * (size_t)nq, b = sizeof(int64_t) * (size_t)(t->nnodes + 1);
     void *ws = nullptr;
-    int rc = workspace(const_cast<ckd_tree *>(t), 0, a + 2 * b + scan_tmp_bytes(t->nnodes) + 256, &ws, s);
+    int rc = sc.take(order_scratch_bytes(t, nq), &ws);
     if (rc) return rc;
     int32_t *lo = (int32_t *)ws, *rk = lo + nq, *ord = rk + nq;
     int64_t *cnt = (int64_t *)((char *)ws + ((a + 255) & ~(size_t)255));
@@ -679,7 +698,8 @@ int order_queries(const ckd_tree *t, const TreeView<T> &tv, const T *q, int64_t 
 }
 
 template <typename T, bool PER>
-int sph_launch(const ckd_tree *t, const T *q, int64_t nq, int k, const T *mass, T *h, T *rho, hipStream_t s) {
+int sph_launch(const ckd_tree *t, const T *q, int64_t nq, int k, const T *mass, T *h, T *rho, Scratch &sc,
+               hipStream_t s) {
     const TreeView<T> tv = view<T>(t);
     if (!stack_ok<T, 3>(t, 64) || t->n >= INT32_MAX) {
         set_error("the SPH step needs a 3-D tree of depth <= 64 and fewer than 2^31 points");
@@ -687,7 +707,7 @@ int sph_launch(const ckd_tree *t, const T *q, int64_t nq, int k, const T *mass, 
     }
     const int32_t *order = nullptr, *leaf_of = nullptr;
     if (nq >= ORDER_MIN_QUERIES && nq < INT32_MAX && t->nnodes > 1) {
-        const int rc = order_queries<T, 3>(t, tv, q, nq, &order, &leaf_of, s);
+        const int rc = order_queries<T, 3>(t, tv, q, nq, &order, &leaf_of, sc, s);
         if (rc) return rc;
     }
     const int2 root = make_int2(0, t->nnodes > 1 ? -1 : (int32_t)t->n);
@@ -724,7 +744,7 @@ int ball_kernel_env() {
 
 template <typename T, int D>
 int ball_launch(const ckd_tree *t, const T *q, int64_t nq, T r2, int64_t *counts, const int64_t *off, T *dist,
-                int64_t *idx, hipStream_t s) {
+                int64_t *idx, Scratch &sc, hipStream_t s) {
     const TreeView<T> tv = view<T>(t);
     const unsigned g = (unsigned)ceil_div(nq, QT);
     const bool fill = off != nullptr;
@@ -734,7 +754,7 @@ int ball_launch(const ckd_tree *t, const T *q, int64_t nq, T r2, int64_t *counts
         if (kind == CKD_KNN_STACK && t->wide && t->depth <= 64 && t->n < INT32_MAX) {
             const int32_t *order = nullptr;
             if (nq >= ORDER_MIN_QUERIES && nq < INT32_MAX && t->nnodes > 1) {
-                const int rc = order_queries<T, D>(t, tv, q, nq, &order, nullptr, s);
+                const int rc = order_queries<T, D>(t, tv, q, nq, &order, nullptr, sc, s);
                 if (rc) return rc;
             }
             const int2 root = make_int2(0, t->nnodes > 1 ? -1 : (int32_t)t->n);
@@ -804,19 +824,20 @@ int box_launch(const ckd_tree *t, const T *b, int64_t nb, int64_t *counts, const
 // params != nullptr asks for learned seeds.
 template <typename T, bool PER>
 int knn_per(const ckd_tree *t, const void *q, int64_t nq, int k, int sorted, const float *seed, const float *params,
-            float margin, int G, const uint16_t *table, void *dist, int64_t *idx, int64_t *stats, hipStream_t s) {
+            float margin, int G, const uint16_t *table, void *dist, int64_t *idx, int64_t *stats, Scratch &sc,
+            hipStream_t s) {
     const Learned ln{params, margin, G, table};
     const Learned *lp = params ? &ln : nullptr;
-    CKD_DIM_SWITCH(t->ndim, (knn_launch<T, D_, PER>(t, (const T *)q, nq, k, sorted, seed, lp, (T *)dist, idx, stats, s)))
+    CKD_DIM_SWITCH(t->ndim, (knn_launch<T, D_, PER>(t, (const T *)q, nq, k, sorted, seed, lp, (T *)dist, idx, stats, sc, s)))
     return CKD_ERR_ARG;
 }
 
 template <typename T>
 int ball_t(const ckd_tree *t, const void *q, int64_t nq, double r, int64_t *counts, const int64_t *off, void *dist,
-           int64_t *idx, hipStream_t s) {
+           int64_t *idx, Scratch &sc, hipStream_t s) {
     const T rr = (T)r;   // radius is a real(prec) dummy (cosmokdtree.f90:1019)
     const T r2 = rr * rr;
-    CKD_DIM_SWITCH(t->ndim, (ball_launch<T, D_>(t, (const T *)q, nq, r2, counts, off, (T *)dist, idx, s)))
+    CKD_DIM_SWITCH(t->ndim, (ball_launch<T, D_>(t, (const T *)q, nq, r2, counts, off, (T *)dist, idx, sc, s)))
     return CKD_ERR_ARG;
 }
 
