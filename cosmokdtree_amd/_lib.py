@@ -54,6 +54,9 @@ def _load():
     lib.ckd_get_info.argtypes = [vp, C.POINTER(TreeInfo)]
     lib.ckd_knn.argtypes = [vp, vp, i64, i32, i32, vp, vp, vp, vp, vp]
     lib.ckd_knn_config.argtypes = [vp, i32, i32, i32]
+    lib.ckd_knn_noalloc.argtypes = [vp, vp, i64, i32, i32, vp, vp, vp, vp, vp]
+    lib.ckd_knn_noalloc_scratch_bytes.argtypes = [vp, i64, i32]
+    lib.ckd_knn_noalloc_scratch_bytes.restype = C.c_size_t
     lib.ckd_sph_density.argtypes = [vp, vp, i64, i32, vp, vp, vp, vp]
     lib.ckd_knn_timing.argtypes = [vp, i32, C.POINTER(C.c_double), C.POINTER(i64)]
     lib.ckd_tree_save.argtypes = [vp, C.c_char_p]

@@ -30,6 +30,7 @@
 #ifndef COSMOKDTREE_AMD_H
 #define COSMOKDTREE_AMD_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -133,6 +134,28 @@ int ckd_knn(const ckd_tree *tree, const void *queries, int64_t nq, int32_t k, in
  */
 int ckd_sph_density(const ckd_tree *tree, const void *queries, int64_t nq, int32_t k, const void *mass, void *h,
                     void *rho, void *stream);
+
+/*
+ * kNN that allocates nothing: the same arguments, kernels and results as
+ * ckd_knn, with the call's scratch (the leaf ordering of the queries, the
+ * re-search lists of seeded calls) taken from one static device array of
+ * CKD_NOALLOC_SCRATCH_BYTES per GPU instead of the library's allocator, and
+ * no events created (ckd_knn_timing does not see its launches).  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Limits: k <= 64, trees below 2^31 - 1
+ * points, and a batch whose scratch (ckd_knn_noalloc_scratch_bytes: about
+ * 12 B per query plus 16 B per tree node, when there are at least 4096
+ * queries; seeds add about 36 B per query) fits the array; anything beyond
+ * returns CKD_ERR_LIMIT before any work is enqueued.  Every call on a GPU
+ * shares that GPU's array: calls that overlap on one GPU (other streams or
+ * host threads) race and must be serialised by the caller.
+ */
+#define CKD_NOALLOC_SCRATCH_BYTES (64 << 20)
+int ckd_knn_noalloc(const ckd_tree *tree, const void *queries, int64_t nq, int32_t k, int32_t sorted, void *dist,
+                    int64_t *idx, const float *seed_r, int64_t *stats, void *stream);
+/* Scratch bytes ckd_knn_noalloc needs for nq queries (seeded != 0: with seeds). */
+size_t ckd_knn_noalloc_scratch_bytes(const ckd_tree *tree, int64_t nq, int32_t seeded);
 
 #define CKD_KNN_AUTO (-1)
 #define CKD_KNN_LANE 0   /* one traversal per query */
