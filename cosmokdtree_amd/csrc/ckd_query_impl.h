@@ -195,7 +195,7 @@ __global__ __launch_bounds__(QT, 3) void k_knn_w(TreeView<T> tv, const T *querie
 // folded away (the group scans cost registers and time even when G is 1 at
 // run time); GC = 0 takes G from the argument.
 template <typename T, int D, bool PER, int KMAX, int SD, int GC>
-__global__ __launch_bounds__(QT, 3) void k_knn_s(TreeView<T> tv, int2 root, const T *queries, int64_t nq,
+__global__ __launch_bounds__(QT, sizeof(T) == 4 ? 4 : 3) void k_knn_s(TreeView<T> tv, int2 root, const T *queries, int64_t nq,
                                               const int64_t *nq_dev, int k, int sorted, const float *seed,
                                               const int32_t *order, const int32_t *leaf_of, T *dout,
                                               int64_t *iout, int64_t *stats, Retry rt, int Grt) {

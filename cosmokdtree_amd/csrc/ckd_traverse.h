@@ -424,53 +424,80 @@ __device__ __forceinline__ void load_rows(const T *pts, int32_t p, T (&x)[PB][D]
         for (int d = 0; d < D; d++) x[j][d] = src[j * PS + d];
 }
 
-// Leaf scan shared by the stack kernels: rows [pb, pe), candidates buffered
-// per lane in LDS (CB > 0 slots per lane, slot j of lane tid at
-// j * nthreads + tid) or inserted directly (CB == 0).  A lane skips the leaf
-// when its rows lie in the lane's prefilled range [xs, xs + xm) (knn_stack):
-// a leaf's rows are either all inside an ancestor's row range or all
-// outside it.
+// Candidates of the stack walk wait in a per-lane LDS buffer (CB > 0 slots
+// per lane, slot j of lane tid at j * nthreads + tid; `pend` of them filled)
+// and enter the list in rounds: round j inserts every lane's j-th pending
+// candidate, re-tested against the list's current k-th distance, so a flush
+// costs as many list updates as the fullest lane holds.
+template <bool NOIDX, typename T, int KMAX, int CB>
+__device__ __forceinline__ void flush_candidates(T (&bd)[KMAX], int32_t (&bs)[KMAX], Counters &c, const T *buf_d,
+                                                 const int32_t *buf_s, int tid, int nthreads, int &pend) {
+    const int jmax = __reduce_max_sync(~0ull, pend);
+#pragma unroll 1
+    for (int j = 0; j < jmax; j++) {
+        if (j < pend) {
+            const T d = buf_d[j * nthreads + tid];
+            if (d < bd[KMAX - 1]) {
+                list_put<NOIDX, T, KMAX>(bd, bs, d, NOIDX ? 0 : buf_s[j * nthreads + tid]);
+                c.inserted++;
+            }
+        }
+    }
+    pend = 0;
+}
+
+// Leaf scan shared by the stack kernels: rows [pb, pe).  With CB > 0 a row
+// closer than the lane's k-th distance joins the lane's pending candidates;
+// they stay pending across leaves and are flushed only when some lane's
+// buffer could overflow on the next block of rows (and once when the walk
+// ends).  Flushing after every leaf instead made each leaf cost the rounds
+// of its busiest lane: most rounds found one or two lanes with a candidate.
+// The bound used meanwhile is the list's, which only shrinks at flushes, so
+// pending candidates can only make the walk visit more, never less.  With
+// CB == 0 rows are inserted directly.  A lane skips the leaf when its rows
+// lie in the lane's prefilled range [xs, xs + xm) (knn_stack): a leaf's rows
+// are either all inside an ancestor's row range or all outside it.
 template <typename T, int D, bool PER, int KMAX, int CB, bool NOIDX = false>
 __device__ __forceinline__ void wave_scan_rows(const TreeView<T> &tv, int32_t pb, int32_t pe, const T (&q)[D],
                                                bool run_q, T (&bd)[KMAX], int32_t (&bs)[KMAX], Counters &c,
                                                T *buf_d, int32_t *buf_s, int tid, int nthreads, int32_t xs,
-                                               uint32_t xm) {
+                                               uint32_t xm, int &pend) {
     constexpr int PB = sizeof(T) == 4 ? 8 : 4;  // rows loaded per batch (CB is a multiple)
     const bool run = run_q && (uint32_t)(pb - xs) >= xm;
     if constexpr (CB > 0) {
-        // leaves longer than CB rows go in chunks of CB, so a lane never
-        // buffers more than CB candidates; candidates are inserted in row
-        // order and re-tested against the current k-th distance
+        // blocks of PB rows; after a block the buffers are flushed only if
+        // some lane could not take a whole next block
 #pragma unroll 1
-        for (int32_t cb = pb; cb < pe; cb += CB) {
-            const int32_t ce = cb + CB < pe ? cb + CB : pe;
+        for (int32_t p = pb; p < pe; p += PB) {
             const T bound = bd[KMAX - 1];
-            int cnt = 0;
-            T *wd = buf_d + tid;  // this lane's next free slot
-            int32_t *ws = buf_s + tid;
-#pragma unroll 1
-            for (int32_t p = cb; p < ce; p += PB) {
-                T x[PB][D];
-                load_rows<T, D, PB>(tv.pts, p, x);
+            int cnt = pend;
+            T *wd = buf_d + cnt * nthreads + tid;  // this lane's next free slot
+            int32_t *ws = buf_s + cnt * nthreads + tid;
+            T x[PB][D];
+            load_rows<T, D, PB>(tv.pts, p, x);
 #pragma unroll
-                for (int j = 0; j < PB; j++) {
-                    const T d = dist2<T, D, PER>(x[j], q, tv.L);
-                    if (run && p + j < ce && d < bound) {
-                        *wd = d;
-                        if constexpr (!NOIDX) *ws = p + j;
-                        wd += nthreads;
-                        ws += nthreads;
-                        cnt++;
-                    }
+            for (int j = 0; j < PB; j++) {
+                const T d = dist2<T, D, PER>(x[j], q, tv.L);
+                if (run && p + j < pe && d < bound) {
+                    *wd = d;
+                    if constexpr (!NOIDX) *ws = p + j;
+                    wd += nthreads;
+                    ws += nthreads;
+                    cnt++;
                 }
             }
+            pend = cnt;
+            if (!__any(cnt > CB - PB)) continue;
+            // the flush, written out here: through flush_candidates the
+            // compiler kept more of the list in scratch
+            pend = 0;
             const int jmax = __reduce_max_sync(~0ull, cnt);
 #pragma unroll 1
             for (int j = 0; j < jmax; j++) {
                 if (j < cnt) {
-                    const T d = buf_d[j * nthreads + tid];
-                    if (d < bd[KMAX - 1]) {
-                        list_put<NOIDX, T, KMAX>(bd, bs, d, NOIDX ? 0 : buf_s[j * nthreads + tid]);
+                    const T dj = buf_d[j * nthreads + tid];
+                    if (dj < bd[KMAX - 1]) {
+                        list_put<NOIDX, T, KMAX>(bd, bs, dj, NOIDX ? 0 : buf_s[j * nthreads + tid]);
                         c.inserted++;
                     }
                 }
@@ -589,7 +616,7 @@ template <typename T, int D, bool PER, int KMAX, int SD, int CB, bool CHECK, boo
 __device__ __forceinline__ bool stack_walk(const TreeView<T> &tv, const T (&q)[D], bool run, T safe2, T (&bd)[KMAX],
                                            int32_t (&bs)[KMAX], int32_t &a, int32_t &b, int &sp, Counters &c,
                                            T *buf_d, int32_t *buf_s, int tid, int nthreads, StackEntry<T, D> *stk,
-                                           int G, int sub, int gbase, int32_t xs, uint32_t xm) {
+                                           int G, int sub, int gbase, int32_t xs, uint32_t xm, int &pend) {
     for (;;) {
         if (CHECK && !__any(run && !(safe2 >= bd[KMAX - 1]))) return false;
         c.visited++;
@@ -597,7 +624,7 @@ __device__ __forceinline__ bool stack_walk(const TreeView<T> &tv, const T (&q)[D
         if (b >= 0) {
             if (G == 1)
                 wave_scan_rows<T, D, PER, KMAX, CB, NOIDX>(tv, a, a + b, q, run, bd, bs, c, buf_d, buf_s, tid, nthreads,
-                                                           xs, xm);
+                                                           xs, xm, pend);
             else group_scan_rows<T, D, PER, KMAX>(tv, a, a + b, q, run, bd, bs, c, G, sub, gbase);
         } else {
             T b0[2 * D], b1[2 * D];
@@ -758,15 +785,19 @@ __device__ __forceinline__ bool knn_stack(const TreeView<T> &tv, int2 root, cons
         }
         c.tested += xm;
     }
+    int pend = 0;  // candidates waiting in this lane's buffer
     if constexpr (PER) {
         const T safe2 = wrap_free_bound<T, D>(q, tv.L);
         if (!stack_walk<T, D, true, KMAX, SD, CB, true, NOIDX>(tv, q, run, safe2, bd, bs, a, b, sp, c, buf_d, buf_s, tid,
-                                                        nthreads, stk, G, sub, gbase, xs, xm))
+                                                        nthreads, stk, G, sub, gbase, xs, xm, pend))
             stack_walk<T, D, false, KMAX, SD, CB, false, NOIDX>(tv, q, run, safe2, bd, bs, a, b, sp, c, buf_d, buf_s, tid,
-                                                         nthreads, stk, G, sub, gbase, xs, xm);
+                                                         nthreads, stk, G, sub, gbase, xs, xm, pend);
     } else {
         stack_walk<T, D, false, KMAX, SD, CB, false, NOIDX>(tv, q, run, T(0), bd, bs, a, b, sp, c, buf_d, buf_s, tid,
-                                                     nthreads, stk, G, sub, gbase, xs, xm);
+                                                     nthreads, stk, G, sub, gbase, xs, xm, pend);
+    }
+    if constexpr (CB > 0) {
+        if (G == 1) flush_candidates<NOIDX, T, KMAX, CB>(bd, bs, c, buf_d, buf_s, tid, nthreads, pend);
     }
     if (!active) return true;
     if (bs[KMAX - 1] < 0 && init != T(FLT_MAX)) {
