@@ -116,6 +116,23 @@ def test_adam_matches_oracle(L_):
     assert np.allclose(model.params.cpu().numpy(), po, rtol=1e-6, atol=1e-7)
 
 
+def test_adam_three_steps_match_oracle(L_):
+    """The GPU Adam kernel over three steps (moments and bias correction
+    carried between calls) against the oracle's three steps."""
+    rng = np.random.default_rng(21)
+    p = M.init_params(6)
+    model = L_.RadiusModel(params=p)
+    po, mo, vo = p.copy(), np.zeros_like(p), np.zeros_like(p)
+    for t in (1, 2, 3):
+        g = rng.standard_normal(M.NPARAM).astype(np.float32)
+        model.grad.copy_(torch.from_numpy(g))
+        model.adam(1e-3)
+        po, mo, vo = M.adam(po, g, mo, vo, t, 1e-3)
+    assert np.allclose(model.m.cpu().numpy(), mo, rtol=1e-5, atol=1e-8)
+    assert np.allclose(model.v.cpu().numpy(), vo, rtol=1e-5, atol=1e-10)
+    assert np.allclose(model.params.cpu().numpy(), po, rtol=1e-6, atol=1e-7)
+
+
 def test_seeded_knn_is_exact(L_, clustered):
     pts, tree = clustered
     q = torch.from_numpy(pts[::37]).cuda()
@@ -133,15 +150,16 @@ def test_seeded_knn_is_exact(L_, clustered):
 
 def test_trained_network_gains(L_, clustered, ckd):
     """Train on an independent point set, then search this one with the
-    learned seeds: the fallback rate is low and the results are the plain
-    ones, for the trained margin and for margins far too short or long.
-    (The walk's own ancestor prefill already gives each lane a tight early
-    bound, so the seed no longer removes many insertions; the counts are
-    printed, not asserted.)"""
+    learned seeds: the fallback rate is low, the results are the plain ones
+    (for the trained margin and for margins far too short or long), and the
+    seeded walk updates its k-lists markedly less often than the plain one
+    (list insertions are most of the traversal's instructions).  An exact
+    seed (the true k-th distance) is the ceiling: exactly k insertions per
+    query (up to ties at the k-th distance)."""
     from cosmokdtree_amd.synthetic import clustered_periodic
     train_pts = clustered_periodic(1 << 18, 100.0, 0.5, seed=77, dtype=np.float32)
     ttree = ckd.kdtree._build(train_pts, 16, np.full(3, 100.0), keep_points=True)
-    model = L_.fit(ttree, 100000, k=32, epochs=3, quantile=0.98)
+    model = L_.fit(ttree, 100000, k=32, epochs=3, quantile=0.9)
     ttree.free()
     pts, tree = clustered
     tree.knn_config("stack", 0, 64)
@@ -157,6 +175,27 @@ def test_trained_network_gains(L_, clustered, ckd):
           f"per query; seeded: {s1[0] / s1[3]:.1f} nodes, {s1[1] / s1[3]:.1f} points, "
           f"{s1[4] / s1[3]:.1f} insertions; fallback {fallback:.4f}")
     assert fallback < 0.1
+    ins_plain, ins_seeded = s0[4] / s0[3], s1[4] / s1[3]
+    assert ins_seeded < 0.9 * ins_plain, (ins_seeded, ins_plain)  # measured 63.8 against 77.9
+    # the one-walk-per-query kernel (no ancestor prefill) gains too
+    tree.knn_config("lane")
+    s3 = torch.zeros(6, dtype=torch.int64, device="cuda")
+    s4 = torch.zeros(6, dtype=torch.int64, device="cuda")
+    tree.knn(q, 32, stats=s3)
+    d4, _ = L_.seeded_knn(tree, model, q, 32, stats=s4)
+    assert torch.equal(d0, d4)
+    s3, s4 = s3.cpu().numpy(), s4.cpu().numpy()
+    assert s4[4] / s4[3] < 0.95 * s3[4] / s3[3], (s4[4] / s4[3], s3[4] / s3[3])  # measured 60.5 against 68.0
+    tree.knn_config("stack", 0, 64)
+    # the ceiling of any network: the exact k-th distance as the seed
+    s2 = torch.zeros(6, dtype=torch.int64, device="cuda")
+    rk = (d0[:, 31].double() * (1 + 2.0 ** -20)).float()
+    d3, _ = tree.knn(q, 32, seed_r=rk, stats=s2)
+    assert torch.equal(d0, d3)
+    s2 = s2.cpu().numpy()
+    print(f"exact seed: {s2[0] / s2[3]:.1f} nodes, {s2[4] / s2[3]:.1f} insertions per query")
+    # k per query, plus the rare point tied with (or within 2^-20 of) the k-th distance
+    assert s2[2] == 0 and 32 * s2[3] <= s2[4] <= 32.001 * s2[3]
     # whatever the margin (nearly every seed short, or all far too long), the
     # results are the plain ones: short seeds go through both retry rounds
     table = L_.feature_table(L_.occupancy(tree))

@@ -105,3 +105,31 @@ def test_gelu_sigmoid_form():
     h = 1e-6
     fd = (M.gelu(z + h) - M.gelu(z - h)) / (2 * h)
     assert np.allclose(M.gelu_grad(z), fd, atol=1e-7)
+
+
+def test_adam_three_steps_closed_form():
+    """Three Adam steps against the moments written out by hand (Kingma & Ba
+    2015, bias-corrected): a constant gradient g gives m_hat = g and
+    v_hat = g^2 at every step, so each step moves by lr * g / (|g| + eps);
+    the gradients g, -g, g give m = 0.1 g, -0.01 g, 0.091 g with m_hat =
+    m / (1 - 0.9^t) and v_hat = g^2."""
+    lr, eps = 1e-2, 1e-8
+    g = np.array([0.5, -2.0, 3e-3], np.float32)
+    p0 = np.array([1.0, -1.0, 0.25], np.float32)
+    p, m, v = p0.copy(), np.zeros(3, np.float32), np.zeros(3, np.float32)
+    for t in (1, 2, 3):
+        p, m, v = M.adam(p, g, m, v, t, lr)
+    assert np.allclose(p, p0 - 3 * lr * g / (np.abs(g) + eps), rtol=0, atol=2e-6)
+    assert np.allclose(m, (1 - 0.9 ** 3) * g, rtol=1e-5)
+    assert np.allclose(v, (1 - 0.999 ** 3) * g * g, rtol=1e-4)
+    p, m, v = p0.copy(), np.zeros(3, np.float32), np.zeros(3, np.float32)
+    want = p0.astype(np.float64)
+    for t, (gt, mt) in enumerate(((g, 0.1), (-g, -0.01), (g, 0.091)), start=1):
+        p, m, v = M.adam(p, gt, m, v, t, lr)
+        mhat = mt * g.astype(np.float64) / (1 - 0.9 ** t)
+        want = want - lr * mhat / (np.abs(g.astype(np.float64)) + eps)
+        assert np.allclose(m, mt * g, rtol=1e-5, atol=1e-9), t
+    assert np.allclose(p, want, rtol=0, atol=2e-6)
+    # a planted mistake (no bias correction) is caught
+    pb = p0.astype(np.float64) - lr * 0.1 * g / (np.sqrt(0.001) * np.abs(g) + eps)
+    assert not np.allclose(pb, p0 - lr * g / (np.abs(g) + eps), atol=2e-6)
