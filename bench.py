@@ -24,6 +24,12 @@ import time
 
 import numpy as np
 
+# the host CPUs this process was given, read before torch loads: with
+# OMP_PROC_BIND set in the environment, torch's OpenMP runtime binds the main
+# thread to one CPU at its first parallel region, and children would inherit
+# that mask.  The host-core baseline's children run on all of them.
+AFFINITY0 = sorted(os.sched_getaffinity(0))
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -50,14 +56,80 @@ def build_bytes_per_point(D, s_c, s_i, n, leaf):
 
 
 def cpu_baseline(n, q, k, cores):
+    """Each implementation in a child process of its own (its own OpenMP
+    runtime and CPU mask), OMP_* set before the runtime loads."""
     env = dict(os.environ)
     env.update(OMP_NUM_THREADS=str(cores), OMP_PROC_BIND="close", OMP_PLACES="cores")
-    cmd = [sys.executable, "-m", "oracle.cpu_baseline", "--n", str(n), "--q", str(q), "--k", str(k)]
-    p = subprocess.run(cmd, cwd=ROOT, env=env, capture_output=True, text=True, timeout=600)
-    if p.returncode != 0:
-        log("cpu baseline failed: " + p.stderr[-2000:])
-        return None
-    return json.loads(p.stdout.strip().splitlines()[-1])
+    out = {}
+    for impl in ("reference", "port"):
+        cmd = [sys.executable, "-m", "oracle.cpu_baseline", "--impl", impl, "--n", str(n), "--q", str(q),
+               "--k", str(k)]
+        p = subprocess.run(cmd, cwd=ROOT, env=env, capture_output=True, text=True, timeout=900,
+                           preexec_fn=lambda: os.sched_setaffinity(0, AFFINITY0))
+        if p.returncode != 0:
+            log(f"cpu baseline ({impl}) failed: " + p.stderr[-2000:])
+            continue
+        r = json.loads(p.stdout.strip().splitlines()[-1])
+        if impl in r:
+            out[impl] = r[impl]
+            out["affinity"] = r["affinity"]
+            out["threads"] = r["threads"]
+    return out or None
+
+
+def _ev_ms(fn, reps=3):
+    """Median HIP-event time of fn() on the current stream (after one warm-up)."""
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ts.append(e0.elapsed_time(e1))
+    return float(np.median(ts))
+
+
+def calibration_table():
+    """GPU rows of the calibration table (device-resident inputs and outputs):
+    config 1 (1e6 uniform points in [0,1)^3, float64, 1e5 queries: build,
+    kNN k=8 sorted, ball r=0.01, box side 0.02 for every query in one call)
+    and the clustered periodic box (2^24 points, float32 and float64: build,
+    self-kNN k=32 sorted, ball r=0.2 on 1e6 of the points)."""
+    import torch
+
+    from cosmokdtree_amd import kdtree
+    from cosmokdtree_amd.synthetic import SEED_POINTS, SEED_QUERIES, SEED_SUBSET, clustered_periodic
+    rows = {}
+    rng = np.random.Generator(np.random.PCG64(SEED_POINTS))
+    P = torch.from_numpy(rng.random((1_000_000, 3))).cuda()
+    Q = torch.from_numpy(np.random.Generator(np.random.PCG64(SEED_QUERIES)).random((100_000, 3))).cuda()
+    tb = _ev_ms(lambda: kdtree._build(P, 16, None).free())
+    tree = kdtree._build(P, 16, None)
+    boxes = torch.stack([Q[:, 0] - 0.01, Q[:, 0] + 0.01, Q[:, 1] - 0.01, Q[:, 1] + 0.01, Q[:, 2] - 0.01,
+                         Q[:, 2] + 0.01], 1).contiguous()
+    rows["c1_f64"] = dict(build_mpts_s=1e6 / tb / 1e3, knn_k8_qps=1e5 / _ev_ms(lambda: tree.knn(Q, 8)) * 1e3,
+                          ball_r001_qps=1e5 / _ev_ms(lambda: tree.ball(Q, 0.01)) * 1e3,
+                          box_side002_qps=1e5 / _ev_ms(lambda: tree.box(boxes)) * 1e3)
+    tree.free()
+    log(f"table c1: {rows['c1_f64']}")
+    n = 1 << 24
+    for tag, dt in (("c2_f32", np.float32), ("c2_f64", np.float64)):
+        P = torch.from_numpy(clustered_periodic(n, 100.0, 0.5, seed=SEED_POINTS, dtype=dt)).cuda()
+        box = np.full(3, 100.0)
+        tb = _ev_ms(lambda: kdtree._build(P, 16, box).free())
+        tree = kdtree._build(P, 16, box)
+        sub = P[torch.from_numpy(np.random.Generator(np.random.PCG64(SEED_SUBSET)).choice(n, 1_000_000,
+                                                                                      replace=False)).cuda()]
+        rows[tag] = dict(build_mpts_s=n / tb / 1e3, knn_k32_qps=n / _ev_ms(lambda: tree.knn(P, 32)) * 1e3,
+                         ball_r02_qps=1e6 / _ev_ms(lambda: tree.ball(sub, 0.2)) * 1e3)
+        tree.free()
+        del P, sub
+        log(f"table {tag}: {rows[tag]}")
+    return rows
 
 
 def main():
@@ -73,8 +145,10 @@ def main():
     ap.add_argument("--train-pairs", type=int, default=2_000_000)
     ap.add_argument("--epochs", type=int, default=4)
     ap.add_argument("--quantile", type=float, default=0.95)
-    ap.add_argument("--cpu-n", type=int, default=1 << 22)
-    ap.add_argument("--cpu-q", type=int, default=1 << 18)
+    ap.add_argument("--cpu-n", type=int, default=1 << 24)
+    ap.add_argument("--cpu-q", type=int, default=1 << 20)
+    ap.add_argument("--table", action="store_true",
+                    help="print the GPU rows of the calibration table (configs 1 and 2) as one JSON line")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device copy of this size (counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -91,6 +165,9 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
+    if args.table:
+        print(json.dumps(calibration_table()), flush=True)
+        return
     dev = torch.device("cuda")
     N, K = args.n, args.k
     L = 100.0
@@ -275,16 +352,20 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, len(os.sched_getaffinity(0)))
-        log(f"cpu baseline on {cores} host threads ...")
+        # every core the process was given, unless OMP_NUM_THREADS sets the share
+        aff = len(AFFINITY0)
+        cores = min(int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or aff, aff)
+        log(f"cpu baseline on {cores} host threads (affinity mask of {aff} CPUs at start, "
+            f"{len(os.sched_getaffinity(0))} now) ...")
         res = cpu_baseline(args.cpu_n, args.cpu_q, K, cores)
-        if res:
+        if res and ("reference" in res or "port" in res):
             main_impl = "reference" if "reference" in res else "port"
             r = res[main_impl]
-            cpu = dict(value=r["knn_qps"], unit="queries/s", cores=cores, kind=main_impl,
+            cpu = dict(value=r["knn_qps"], unit="queries/s", cores=cores, affinity=aff, kind=main_impl,
                        sample=f"clustered periodic recipe, n={args.cpu_n} float32 points, leaf 16: build, then "
-                              f"kNN k={K} sorted of {args.cpu_q} sampled points (OpenMP over queries)",
-                       build_mpts_s=r["build_mpts_s"])
+                              f"kNN k={K} sorted of {args.cpu_q} of the points (OpenMP over queries), each "
+                              f"implementation in its own process",
+                       build_mpts_s=r["build_mpts_s"], build_s=r["build_s"], knn_s=r["knn_s"])
             if "port" in res and main_impl != "port":
                 cpu["port_value"] = res["port"]["knn_qps"]
                 cpu["port_build_mpts_s"] = res["port"]["build_mpts_s"]
