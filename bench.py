@@ -48,6 +48,19 @@ def knn_bytes_per_query(D, s_c, s_i, k, n, nnodes, q):
     return D * s_c + k * (s_c + 8) + (n * (D * s_c + s_i) + nnodes * 16) / q
 
 
+KNN_SOURCES = ("ckd_traverse.h", "ckd_query_impl.h", "ckd_internal.h", "ckd_knn_f32_per.hip")
+
+
+def knn_source_sha():
+    """Hash of the sources the kNN traversal kernel is compiled from."""
+    import hashlib
+    h = hashlib.sha256()
+    for f in KNN_SOURCES:
+        with open(os.path.join(ROOT, "cosmokdtree_amd", "csrc", f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
 def build_bytes_per_point(D, s_c, s_i, n, leaf):
     """Algorithmic HBM bytes per point of the level-synchronous build: input
     read, read+write of coordinates and index per level, leaf-bucket write."""
@@ -371,13 +384,17 @@ def main():
                 cpu["port_build_mpts_s"] = res["port"]["build_mpts_s"]
             log(f"cpu: {json.dumps(cpu)}")
 
+    # HBM bytes per launch from the committed rocprofv3 counter passes, used
+    # only while the kernel sources are the ones they measured
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "knn_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             tr = json.load(f)
-        if tr.get("n") == N and tr.get("k") == K:
+        if tr.get("n") == N and tr.get("k") == K and tr.get("source_sha") == knn_source_sha():
             traffic = tr.get("seeded_bytes_per_launch" if use_learned else "plain_bytes_per_launch")
+        else:
+            log("profiles/knn_traffic.json measured other kernel sources or sizes: traffic not reported")
 
     out = {
         "metric": METRIC,

@@ -1,10 +1,18 @@
 """Compile the gfx950 library ``libcosmokdtree_amd.so`` in-tree with hipcc.
 
     python -m cosmokdtree_amd.build          # or: python cosmokdtree_amd/build.py
+    python -m cosmokdtree_amd.build --debug
 
 Every ``csrc/*.hip`` file is compiled to an object in ``_build/`` (in
 parallel) and linked into ``cosmokdtree_amd/libcosmokdtree_amd.so``.
 Always recompiles (the build check runs this on every call).
+
+``--debug`` builds ``libcosmokdtree_amd_debug.so`` (objects in
+``_build_debug/``): device code with the CKD_DEBUG bounds checks (a failed
+check prints its condition and stops the kernel), host code under
+AddressSanitizer and UndefinedBehaviorSanitizer (``-Xarch_host``, so no
+device code is instrumented).  Load it with ``CKD_LIB`` and the clang ASan
+runtime preloaded (``asan_runtime()``).
 """
 from __future__ import annotations
 
@@ -34,24 +42,43 @@ FLAGS = [
 ]
 
 
-def _compile(src: str, obj_dir: str = OBJ) -> tuple[str, str]:
+DEBUG_FLAGS = ["-DCKD_DEBUG", "-g",
+               "-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+               "-Xarch_host", "-fno-sanitize-recover=all", "-Xarch_host", "-fno-omit-frame-pointer"]
+DEBUG_LIB = os.path.join(HERE, "libcosmokdtree_amd_debug.so")
+
+
+def asan_runtime() -> str:
+    """The clang AddressSanitizer runtime the debug library's host code needs
+    (to be preloaded into the process that loads it)."""
+    p = subprocess.run([HIPCC.replace("hipcc", "amdclang++"), "-print-file-name=libclang_rt.asan-x86_64.so"],
+                       capture_output=True, text=True)
+    path = p.stdout.strip()
+    if p.returncode != 0 or not os.path.isabs(path) or not os.path.exists(path):
+        raise RuntimeError("the clang AddressSanitizer runtime was not found")
+    return path
+
+
+def _compile(src: str, obj_dir: str = OBJ, extra=()) -> tuple[str, str]:
     obj = os.path.join(obj_dir, os.path.basename(src).replace(".hip", ".o"))
-    cmd = [HIPCC, *FLAGS, "-c", src, "-o", obj]
+    cmd = [HIPCC, *FLAGS, *extra, "-c", src, "-o", obj]
     p = subprocess.run(cmd, capture_output=True, text=True)
     if p.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{p.stderr[-6000:]}")
     return obj, p.stderr
 
 
-def build(verbose: bool = False, out: str | None = None, objdir: str | None = None) -> str:
-    """Compile and link; `out`/`objdir` redirect the library and objects."""
-    obj_dir = objdir or OBJ
-    lib_path = out or LIB
+def build(verbose: bool = False, out: str | None = None, objdir: str | None = None, debug: bool = False) -> str:
+    """Compile and link; `out`/`objdir` redirect the library and objects;
+    debug builds the checked variant (module docstring)."""
+    obj_dir = objdir or (OBJ + "_debug" if debug else OBJ)
+    lib_path = out or (DEBUG_LIB if debug else LIB)
+    extra = DEBUG_FLAGS if debug else []
     os.makedirs(obj_dir, exist_ok=True)
     srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
     t0 = time.time()
     with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
-        results = list(ex.map(lambda f: _compile(f, obj_dir), srcs))
+        results = list(ex.map(lambda f: _compile(f, obj_dir, extra), srcs))
     objs = [o for o, _ in results]
     if verbose:
         for (o, err) in results:
@@ -59,7 +86,8 @@ def build(verbose: bool = False, out: str | None = None, objdir: str | None = No
             if errs:
                 print(errs, file=sys.stderr)
     tmp = lib_path + ".tmp"
-    p = subprocess.run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp],
+    link_extra = ["-Xarch_host", "-fsanitize=address,undefined", "-shared-libsan"] if debug else []
+    p = subprocess.run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *link_extra, *objs, "-o", tmp],
                        capture_output=True, text=True)
     if p.returncode != 0:
         raise RuntimeError(f"link failed:\n{p.stderr[-4000:]}")
@@ -70,4 +98,4 @@ def build(verbose: bool = False, out: str | None = None, objdir: str | None = No
 
 
 if __name__ == "__main__":
-    build(verbose=True)
+    build(verbose=True, debug="--debug" in sys.argv[1:])

@@ -83,7 +83,24 @@ struct TreeView {
     const int32_t *wparent;  // parent of each inner node, by wide id (-1 for roots)
     int idx64;
     T L[CKD_MAX_DIM];  // box size per axis (periodic only)
+    int64_t nrows;     // points (rows of pts and idx; pts has PTS_PAD more)
+    int64_t nnodes, nwide;
 };
+
+// Bounds checks of the debug build (python -m cosmokdtree_amd.build --debug
+// compiles with -DCKD_DEBUG): a failed check prints where and stops the
+// kernel.  They cost nothing in the normal build.
+#ifdef CKD_DEBUG
+#define CKD_DASSERT(c)                                                               \
+    do {                                                                             \
+        if (!(c)) {                                                                  \
+            printf("CKD_DEBUG check failed: %s at %s:%d\n", #c, __FILE__, __LINE__);  \
+            __builtin_trap();                                                        \
+        }                                                                            \
+    } while (0)
+#else
+#define CKD_DASSERT(c) ((void)0)
+#endif
 
 // ---------------------------------------------------------------- errors
 void set_error(const char *fmt, ...);
@@ -318,6 +335,9 @@ TreeView<T> view(const ckd_tree *t) {
     v.wparent = t->wparent;
     v.idx64 = t->idx_bits == 64;
     for (int i = 0; i < CKD_MAX_DIM; i++) v.L[i] = (T)t->L[i];
+    v.nrows = t->n;
+    v.nnodes = t->nnodes;
+    v.nwide = t->wide ? t->nwide : 0;
     return v;
 }
 int build_tree(ckd_tree *t, const void *points, hipStream_t s);

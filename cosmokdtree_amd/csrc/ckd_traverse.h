@@ -186,6 +186,7 @@ __device__ __forceinline__ void knn_one(const TreeView<T> &tv, const T (&q)[D], 
     for (int j = 0; j < KMAX; j++) {
         if (j >= KMAX - k) {
             const int32_t s = bs[j];
+            CKD_DASSERT(s >= -1 && s < tv.nrows);
             const int64_t oi = s >= 0 ? load_idx(tv.idx, tv.idx64, s) - idx_base : (int64_t)CKD_EMPTY_SLOT;
             const T dv = sqrt_rn<T>(bd[j]);
             fix |= (dv == pd && oi < pi);
@@ -351,6 +352,7 @@ __device__ __forceinline__ bool knn_wave(const TreeView<T> &tv, const T (&q)[D],
     for (int j = 0; j < KMAX; j++) {
         if (j >= KMAX - k) {
             const int32_t s = bs[j];
+            CKD_DASSERT(s >= -1 && s < tv.nrows);
             const int64_t oi = s >= 0 ? load_idx(tv.idx, tv.idx64, s) - idx_base : (int64_t)CKD_EMPTY_SLOT;
             const T dv = sqrt_rn<T>(bd[j]);
             fix |= (dv == pd && oi < pi);
@@ -474,11 +476,13 @@ __device__ __forceinline__ void wave_scan_rows(const TreeView<T> &tv, int32_t pb
             T *wd = buf_d + cnt * nthreads + tid;  // this lane's next free slot
             int32_t *ws = buf_s + cnt * nthreads + tid;
             T x[PB][D];
+            CKD_DASSERT(p >= 0 && p + PB <= tv.nrows + PTS_PAD);
             load_rows<T, D, PB>(tv.pts, p, x);
 #pragma unroll
             for (int j = 0; j < PB; j++) {
                 const T d = dist2<T, D, PER>(x[j], q, tv.L);
                 if (run && p + j < pe && d < bound) {
+                    CKD_DASSERT(cnt < CB);
                     *wd = d;
                     if constexpr (!NOIDX) *ws = p + j;
                     wd += nthreads;
@@ -564,6 +568,7 @@ __device__ __forceinline__ void knn_write_row(const TreeView<T> &tv, const T (&b
     for (int j = 0; j < KMAX; j++) {
         if (j >= KMAX - k) {
             const int32_t s = bs[j];
+            CKD_DASSERT(s >= -1 && s < tv.nrows);
             const int64_t oi = s >= 0 ? load_idx(tv.idx, tv.idx64, s) - idx_base : (int64_t)CKD_EMPTY_SLOT;
             const T dv = sqrt_rn<T>(bd[j]);
             fix |= (dv == pd && oi < pi);
@@ -629,6 +634,7 @@ __device__ __forceinline__ bool stack_walk(const TreeView<T> &tv, const T (&q)[D
         } else {
             T b0[2 * D], b1[2 * D];
             int4 r;
+            CKD_DASSERT(a >= 0 && a < tv.nwide);
             load_wide<T, D>(tv.wide, a, b0, b1, r);
             const T bound = bd[KMAX - 1];
             const T m0 = mindist2<T, D, PER>(b0, q, tv.L), m1 = mindist2<T, D, PER>(b1, q, tv.L);
@@ -637,6 +643,7 @@ __device__ __forceinline__ bool stack_walk(const TreeView<T> &tv, const T (&q)[D
             if (n0 && n1) {
                 // both: enter first the child more lanes are nearer to, push the other
                 const bool first1 = __popcll(__ballot(run && m1 < m0)) > __popcll(__ballot(run && m0 < m1));
+                CKD_DASSERT(sp < SD);
                 StackEntry<T, D> &e = stk[sp];
 #pragma unroll
                 for (int d = 0; d < 2 * D; d++) e.box[d] = first1 ? b0[d] : b1[d];
@@ -768,6 +775,7 @@ __device__ __forceinline__ bool knn_stack(const TreeView<T> &tv, int2 root, cons
 #pragma unroll 1
         for (int r = 0; r < mmax; r += PB) {
             T x[PB][D];
+            CKD_DASSERT((r < (int)xm ? xs + r : 0) + PB <= tv.nrows + PTS_PAD);
             const T *src = tv.pts + (int64_t)(r < (int)xm ? xs + r : 0) * pstride(D);
 #pragma unroll
             for (int j = 0; j < PB; j++)
@@ -1037,6 +1045,7 @@ __device__ __forceinline__ int64_t ball_stack(const TreeView<T> &tv, int2 root, 
         if (in) {
             cnt += n;
             if (FILL) {
+                CKD_DASSERT(qn + __popcll(m) <= EMIT_CAP);
                 EmitEntry &E = eq[qn + __popcll(m & below)];
                 E.start = s;
                 E.count = n;
@@ -1067,6 +1076,7 @@ __device__ __forceinline__ int64_t ball_stack(const TreeView<T> &tv, int2 root, 
 #pragma unroll 1
             for (int32_t p0 = a; p0 < a + b; p0 += PB) {
                 T x[PB][D];
+                CKD_DASSERT(p0 >= 0 && p0 + PB <= tv.nrows + PTS_PAD);
                 load_rows<T, D, PB>(tv.pts, p0, x);
 #pragma unroll
                 for (int j = 0; j < PB; j++) {
@@ -1086,6 +1096,7 @@ __device__ __forceinline__ int64_t ball_stack(const TreeView<T> &tv, int2 root, 
         } else {
             T b0[2 * D], b1[2 * D];
             int4 r;
+            CKD_DASSERT(a >= 0 && a < tv.nwide);
             load_wide<T, D>(tv.wide, a, b0, b1, r);
             const bool reach0 = run && !(mindist2<T, D, PER>(b0, q, tv.L) > r2);
             const bool reach1 = run && !(mindist2<T, D, PER>(b1, q, tv.L) > r2);
@@ -1103,6 +1114,7 @@ __device__ __forceinline__ int64_t ball_stack(const TreeView<T> &tv, int2 root, 
             const bool t0 = reach0 && !whole0, t1 = reach1 && !whole1;
             const uint64_t m0 = __ballot(t0), m1 = __ballot(t1);
             if (m0 && m1) {  // left first, the right one deferred with its lanes
+                CKD_DASSERT(sp < SD);
                 stk[sp].a = r.z;
                 stk[sp].b = r.w;
                 stk[sp].mask = m1;
@@ -1177,6 +1189,7 @@ __device__ __forceinline__ int64_t box_one(const TreeView<T> &tv, const T (&bx)[
     int64_t cnt = 0, w = 0;
     int32_t cur = root, from = -1;
     while (true) {
+        CKD_DASSERT(cur >= 0 && cur < tv.nnodes);
         const NodeRec<T> nr = tv.nodes[cur];
         if (from == -1) {
             T b[2 * D];

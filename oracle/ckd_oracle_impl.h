@@ -633,6 +633,7 @@ static void T(res_fetch)(const T(ores) *res, const int64_t *offsets, REAL *dout,
     int64_t qi;
     for (qi = 0; qi < res->nq; qi++) {
         const T(olist) *l = &res->lists[qi];
+        if (l->n == 0) continue; /* an empty row has no buffers (memcpy from NULL is undefined) */
         if (dout) memcpy(dout + offsets[qi], l->d, (size_t)l->n * sizeof(REAL));
         memcpy(iout + offsets[qi], l->i, (size_t)l->n * sizeof(int64_t));
     }

@@ -31,7 +31,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-ORACLE_SO = os.path.join(HERE, "libckd_oracle.so")
+# CKD_ORACLE_SO selects another build of the same source (the sanitizer build)
+ORACLE_SO = os.environ.get("CKD_ORACLE_SO") or os.path.join(HERE, "libckd_oracle.so")
 REF_DIR = os.path.join(HERE, "_ref")
 
 _vp = C.c_void_p
