@@ -6,7 +6,8 @@ batch, k and ball radius):
 * leaf size {8, 16, 32, 64}: the forest is rebuilt for each;
 * queries per wave {1, 4, 16, 64} of the stack kNN walk (how many queries
   share one wave; each query is held by 64 / that many lanes), plus the
-  one-walk-per-query kernel;
+  one-walk-per-query kNN kernel;
+* the ball kernel: one walk per query, or one shared walk per wave;
 * stack depth of the stack walk: the smallest of {32, 64} that holds the
   forest's depth.  The stack only sizes an LDS array; a deeper one than the
   tree needs was measured equal within 1 % on the 2e8-point workload, so it
@@ -87,8 +88,8 @@ def save(key: str, best: dict, path: str | None = None) -> None:
     path = path or cache_path()
     os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
     data = _load_cache(path)
-    data[key] = {k: best[k] for k in ("leaf", "kernel", "queries_per_wave", "stack_depth", "ms", "knn_ms", "ball_ms")
-                 if k in best}
+    data[key] = {k: best[k] for k in ("leaf", "kernel", "queries_per_wave", "stack_depth", "ball_kernel", "ms",
+                                      "knn_ms", "ball_ms") if k in best}
     tmp = path + ".tmp"
     with open(tmp, "w") as f:
         json.dump(data, f, indent=1, sort_keys=True)
@@ -116,7 +117,7 @@ def tuned_forest_and_plan(points, offsets, nq_per_set, k_per_set, radius_per_set
     if pick:
         plan = batch.BatchPlan(forest, nq_per_set, k_per_set, radius_per_set, sorted=sorted, threads=threads,
                                knn_kernel=pick["kernel"], queries_per_wave=int(pick["queries_per_wave"]),
-                               stack_depth=int(pick["stack_depth"]))
+                               stack_depth=int(pick["stack_depth"]), ball_kernel=pick.get("ball_kernel", "lane"))
     else:
         plan = batch.BatchPlan(forest, nq_per_set, k_per_set, radius_per_set, sorted=sorted, threads=threads)
     return forest, plan, pick
@@ -152,19 +153,26 @@ def tune_batch(points, offsets, queries, qoffsets, k, radius, boxsize, leaves=LE
         info = forest.info()
         if log:
             log(f"leaf {leaf}: forest of {info.nsets} sets built in {info.build_ms:.1f} ms, depth {info.depth}")
-        # the ball phase does not depend on the kNN settings: timed once per leaf
-        ball_ms = None
+        # the ball phase does not depend on the kNN settings: timed once per
+        # leaf size, for both ball kernels (the faster is kept)
+        ball_ms, ball_kernel = None, "lane"
         if radius is not None:
-            bplan = batch.BatchPlan(forest, nq, np.zeros(len(nq), np.int32), radius, threads=threads)
-            ball_ms = _time_plan(bplan, q, reps)
-            del bplan
+            for bk in ("lane", "stack"):
+                bplan = batch.BatchPlan(forest, nq, np.zeros(len(nq), np.int32), radius, threads=threads,
+                                        ball_kernel=bk)
+                ms = _time_plan(bplan, q, reps)
+                if log:
+                    log(f"leaf {leaf}: ball work {ms:.2f} ms with the {bk} kernel")
+                if ball_ms is None or ms < ball_ms:
+                    ball_ms, ball_kernel = ms, bk
+                del bplan
         plan = batch.BatchPlan(forest, nq, k, None, sorted=True, threads=threads)
         configs = [("stack", w, s) for w in qpws for s in stacks] + ([("lane", 64, 0)] if lane else [])
         for kernel, qpw, sd in configs:
             if sd == 0:
                 sd = 32 if info.depth <= 32 else 64
             row = dict(leaf=leaf, kernel=kernel, queries_per_wave=qpw, stack_depth=sd, depth=int(info.depth),
-                       build_ms=float(info.build_ms), ball_ms=ball_ms)
+                       build_ms=float(info.build_ms), ball_ms=ball_ms, ball_kernel=ball_kernel)
             if kernel == "stack" and sd < info.depth:
                 row["knn_ms"] = row["ms"] = None
                 row["skipped"] = "stack shorter than the tree"

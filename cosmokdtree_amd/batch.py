@@ -30,7 +30,9 @@ lib.ckd_batch_run.argtypes = [_vp_t] * 8
 lib.ckd_batch_status.argtypes = [_vp_t, _vp_t, C.POINTER(C.c_int64), C.POINTER(C.c_int32), _vp_t]
 lib.ckd_batch_free.argtypes = [_vp_t]
 lib.ckd_batch_config.argtypes = [_vp_t, C.c_int32, C.c_int32, C.c_int32]
-for _n in ("ckd_batch_plan", "ckd_batch_run", "ckd_batch_status", "ckd_batch_free", "ckd_batch_config"):
+lib.ckd_batch_config_ball.argtypes = [_vp_t, C.c_int32]
+for _n in ("ckd_batch_plan", "ckd_batch_run", "ckd_batch_status", "ckd_batch_free", "ckd_batch_config",
+           "ckd_batch_config_ball"):
     getattr(lib, _n).restype = C.c_int
 
 
@@ -86,8 +88,9 @@ class BatchPlan:
 
     def __init__(self, forest: KDTree, nq_per_set, k_per_set, radius_per_set=None, sorted: bool = True,
                  threads: int = 256, blocks: int = 0, ball_capacity: int | None = None, knn_kernel: str | None = None,
-                 queries_per_wave: int = 64, stack_depth: int = 0):
+                 queries_per_wave: int = 64, stack_depth: int = 0, ball_kernel: str = "lane"):
         self.forest = forest
+        self.ball_kernel = ball_kernel
         self.knn_kernel, self.queries_per_wave, self.stack_depth = knn_kernel, int(queries_per_wave), int(stack_depth)
         nq = np.asarray(nq_per_set, dtype=np.int64)
         self.qoff = np.zeros(len(nq) + 1, dtype=np.int64)
@@ -132,20 +135,26 @@ class BatchPlan:
         self.ball_idx = torch.empty(max(self.capacity, 1), dtype=torch.int64, device=dev)
 
     def _apply_config(self):
+        kinds = {"lane": 0, "stack": 2}
+        if self.ball_kernel not in kinds:
+            raise ValueError("ball_kernel must be 'lane' or 'stack'")
+        check(lib.ckd_batch_config_ball(self._plan, kinds[self.ball_kernel]), "batch_config_ball")
         if self.knn_kernel is None:  # the plan's own default (by query density)
             return
-        kinds = {"lane": 0, "stack": 2}
         if self.knn_kernel not in kinds:
             raise ValueError("knn_kernel must be 'lane' or 'stack'")
         check(lib.ckd_batch_config(self._plan, kinds[self.knn_kernel], self.queries_per_wave, self.stack_depth),
               "batch_config")
 
-    def config(self, knn_kernel: str = "stack", queries_per_wave: int = 64, stack_depth: int = 0):
+    def config(self, knn_kernel: str = "stack", queries_per_wave: int = 64, stack_depth: int = 0,
+               ball_kernel: str | None = None):
         """kNN kernel ("stack": shared walks, queries in leaf order, or
         "lane": one walk per query), queries per wave of the stack walk
-        (1, 4, 16, 64) and its stack depth (0, 32, 64).  Results do not
-        depend on it."""
+        (1, 4, 16, 64) and its stack depth (0, 32, 64); ball kernel ("lane"
+        or "stack", the shared walk).  Results do not depend on it."""
         self.knn_kernel, self.queries_per_wave, self.stack_depth = knn_kernel, int(queries_per_wave), int(stack_depth)
+        if ball_kernel is not None:
+            self.ball_kernel = ball_kernel
         self._apply_config()
         self._graph = None
         return self

@@ -158,6 +158,15 @@ int ckd_batch_config(ckd_batch *b, int32_t knn_kernel, int32_t queries_per_wave,
     return CKD_OK;
 }
 
+int ckd_batch_config_ball(ckd_batch *b, int32_t ball_kernel) {
+    if (!b || (ball_kernel != CKD_KNN_LANE && ball_kernel != CKD_KNN_STACK)) {
+        set_error("ckd_batch_config_ball: kernel CKD_KNN_LANE or CKD_KNN_STACK");
+        return CKD_ERR_ARG;
+    }
+    b->ball_kernel = ball_kernel;
+    return CKD_OK;
+}
+
 int ckd_batch_run(ckd_batch *b, const void *queries, void *knn_dist, int64_t *knn_idx, int64_t *ball_off,
                   void *ball_dist, int64_t *ball_idx, void *stream) {
     if (!b || (b->nq > 0 && !queries) || (b->knn_total > 0 && (!knn_dist || !knn_idx)) ||

@@ -40,6 +40,7 @@ struct ckd_batch {
     int64_t nsets = 0, nq = 0, knn_total = 0, capacity = 0;
     int sorted = 1, nt = 256, grid = 1024, has_ball = 0, has_knn = 0;
     int knn_kernel = CKD_KNN_STACK, qpw = 64, stack = 0;  // set by ckd_batch_plan / ckd_batch_config
+    int ball_kernel = CKD_KNN_LANE;  // ball items: one walk per query, or CKD_KNN_STACK (ckd_batch_config_ball)
     // items_a: kNN (k_batch); items_c: ball counts, items_b: ball fills (k_batch_ball)
     BatchItem *items_a = nullptr, *items_b = nullptr, *items_c = nullptr;
     int64_t n_a = 0, n_b = 0, n_c = 0;
@@ -256,6 +257,62 @@ __global__ __launch_bounds__(NT) void k_batch_ball(TreeView<T> tv, const int64_t
     }
 }
 
+// The ball items with one walk per wave (ball_stack) over each set's wide
+// records: the item's queries are in the set's leaf order and its radius is
+// the set's, so a wave's 64 lanes share their walk.  Every lane of a wave
+// takes part (a lane past the item's queries is inactive), because the walk
+// is wave-cooperative: its votes, and the drain of whole-subtree rows,
+// which spreads each queued range over all 64 lanes.  Each wave has its own
+// LDS stack and emit queue, shared with nothing else.  A lane whose rows
+// would pass the ball capacity writes nothing and raises the overflow flag.
+template <typename T, int D, bool PER, int NT, int SD>
+__global__ __launch_bounds__(NT) void k_batch_ball_s(TreeView<T> tv, const int64_t *set_off, const T *queries,
+                                                     const BatchItem *items, int64_t nitems, int64_t *head,
+                                                     const double *rset, const int32_t *order, int64_t *ball_counts,
+                                                     const int64_t *ball_off, T *bdist, int64_t *bidx,
+                                                     int64_t capacity, int64_t *overflow) {
+    __shared__ int64_t s_item;
+    __shared__ BallEntry stk[NT / WAVE][SD];
+    __shared__ EmitEntry eq[NT / WAVE][EMIT_CAP];
+    const int wv = threadIdx.x / WAVE;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = (int64_t)atomicAdd((unsigned long long *)head, 1ull);
+        __syncthreads();
+        const int64_t it = s_item;
+        __syncthreads();
+        if (it >= nitems) break;
+        const BatchItem w = items[it];
+        if ((int)(threadIdx.x & ~(WAVE - 1)) >= w.nq) continue;  // whole wave without queries
+        const bool has = (int)threadIdx.x < w.nq;
+        const int64_t qg = has ? (order ? (int64_t)order[w.q0 + threadIdx.x] : w.q0 + threadIdx.x) : 0;
+        T q[D];
+#pragma unroll
+        for (int d = 0; d < D; d++) q[d] = has ? queries[qg * D + d] : T(0);
+        const int64_t base = set_off[w.set];
+        const T r = (T)rset[w.set];
+        const T r2 = r * r;
+        const int2 rp = tv.wroot[w.set];
+        const int2 rows = rp.y >= 0 ? rp : tv.wrange[rp.x];
+        const T *rb = tv.dbox + (int64_t)w.set * 2 * D;
+        if (w.kind == 1) {
+            const int64_t c = ball_stack<T, D, PER, false, SD>(tv, rp, rb, rows, q, has, r2, nullptr, nullptr, 0, base,
+                                                               stk[wv], eq[wv]);
+            if (has) ball_counts[qg] = c;
+        } else {
+            bool act = has;
+            int64_t o = 0;
+            if (has) {
+                o = ball_off[qg];
+                if (ball_off[qg + 1] > capacity) {
+                    act = false;
+                    atomicOr((unsigned long long *)overflow, 1ull);
+                }
+            }
+            ball_stack<T, D, PER, true, SD>(tv, rp, rb, rows, q, act, r2, bdist, bidx, o, base, stk[wv], eq[wv]);
+        }
+    }
+}
+
 template <typename T, int D, bool PER, int NT, int SD>
 int run_t(ckd_batch *b, const T *queries, T *kdist, int64_t *kidx, int64_t *ball_off, T *bdist, int64_t *bidx,
           hipStream_t s) {
@@ -292,16 +349,34 @@ int run_t(ckd_batch *b, const T *queries, T *kdist, int64_t *kidx, int64_t *ball
     }
     CKD_HIP(hipGetLastError());
     if (b->has_ball) {
-        if (b->n_c > 0)  // ball counts
-            k_batch_ball<T, D, PER, NT><<<b->grid, NT, 0, s>>>(tv, t->set_off, queries, b->items_c, b->n_c,
-                                                               b->counters + 3, b->rset, order, b->counts, ball_off,
-                                                               bdist, bidx, b->capacity, b->counters + 2, step_limit);
+        // the shared walk needs the wide records, queries in leaf order and a
+        // stack that holds the forest's depth
+        const bool ball_stack_walk = b->ball_kernel == CKD_KNN_STACK && tv.wparent && order && t->depth <= SD;
+        if (b->n_c > 0) {  // ball counts
+            if (ball_stack_walk)
+                k_batch_ball_s<T, D, PER, NT, SD><<<b->grid, NT, 0, s>>>(tv, t->set_off, queries, b->items_c, b->n_c,
+                                                                         b->counters + 3, b->rset, order, b->counts,
+                                                                         ball_off, bdist, bidx, b->capacity,
+                                                                         b->counters + 2);
+            else
+                k_batch_ball<T, D, PER, NT><<<b->grid, NT, 0, s>>>(tv, t->set_off, queries, b->items_c, b->n_c,
+                                                                   b->counters + 3, b->rset, order, b->counts, ball_off,
+                                                                   bdist, bidx, b->capacity, b->counters + 2,
+                                                                   step_limit);
+        }
         int rc = scan_i64(b->counts, ball_off, b->nq, true, b->scan_tmp, s);
         if (rc) return rc;
         if (b->n_b > 0) {  // ball fills
-            k_batch_ball<T, D, PER, NT><<<b->grid, NT, 0, s>>>(tv, t->set_off, queries, b->items_b, b->n_b,
-                                                               b->counters + 1, b->rset, order, b->counts, ball_off,
-                                                               bdist, bidx, b->capacity, b->counters + 2, step_limit);
+            if (ball_stack_walk)
+                k_batch_ball_s<T, D, PER, NT, SD><<<b->grid, NT, 0, s>>>(tv, t->set_off, queries, b->items_b, b->n_b,
+                                                                         b->counters + 1, b->rset, order, b->counts,
+                                                                         ball_off, bdist, bidx, b->capacity,
+                                                                         b->counters + 2);
+            else
+                k_batch_ball<T, D, PER, NT><<<b->grid, NT, 0, s>>>(tv, t->set_off, queries, b->items_b, b->n_b,
+                                                                   b->counters + 1, b->rset, order, b->counts, ball_off,
+                                                                   bdist, bidx, b->capacity, b->counters + 2,
+                                                                   step_limit);
             // the fill walks wrote d^2; rows past the capacity were not written
             k_sqrt_rows<T><<<SQRT_BLOCKS, 256, 0, s>>>(bdist, ball_off + b->nq, b->capacity);
         }

@@ -99,7 +99,39 @@ std::string check_pair(int2 p, int64_t nwide, int64_t n) {
     return p.x >= 0 && (int64_t)p.x + p.y <= n ? "" : "leaf rows out of range";
 }
 
-std::string check_array(int which, const char *raw, size_t bytes, const PersistHeader &h, int32_t &depth) {
+// Child pairs of the wide records (array 6), kept for the check of the roots
+// (array 7): every wide record is reached once from the set roots, and the
+// deepest chain of inner nodes is no deeper than the node links say (the
+// stack walks size their stacks by that depth).
+struct WideLinks {
+    std::vector<int4> refs;
+};
+
+std::string check_wide_tree(const WideLinks &wl, const int2 *roots, int64_t nsets, int32_t depth) {
+    const int64_t nw = (int64_t)wl.refs.size();
+    std::vector<uint8_t> seen(nw, 0);
+    std::vector<std::pair<int32_t, int32_t>> stk;  // (wide id, inner nodes above it)
+    int64_t reached = 0;
+    for (int64_t r = 0; r < nsets; r++) {
+        if (roots[r].y >= 0) continue;  // a root that is a leaf
+        stk.push_back({roots[r].x, 0});
+        while (!stk.empty()) {
+            const auto [w, lv] = stk.back();
+            stk.pop_back();
+            if (seen[w]) return "a wide record reached twice";
+            seen[w] = 1;
+            reached++;
+            if (lv + 1 > depth) return "wide records deeper than the node links";
+            const int4 f = wl.refs[w];
+            if (f.y < 0) stk.push_back({f.x, lv + 1});
+            if (f.w < 0) stk.push_back({f.z, lv + 1});
+        }
+    }
+    return reached == nw ? "" : "wide records unreachable from the roots";
+}
+
+std::string check_array(int which, const char *raw, size_t bytes, const PersistHeader &h, int32_t &depth,
+                        WideLinks &wl) {
     const int64_t n = h.n, nn = h.nnodes;
     switch (which) {
         case 0:
@@ -119,11 +151,16 @@ std::string check_array(int which, const char *raw, size_t bytes, const PersistH
         }
         case 6: {
             const size_t wb = wide_bytes_rt(h.ndim, h.prec);
+            wl.refs.resize((size_t)h.nwide);
             for (int64_t w = 0; w < h.nwide; w++) {
                 const int4 ref = *(const int4 *)(raw + w * wb + 4 * (size_t)h.ndim * h.prec);
                 std::string e = check_pair(make_int2(ref.x, ref.y), h.nwide, n);
                 if (e.empty()) e = check_pair(make_int2(ref.z, ref.w), h.nwide, n);
                 if (!e.empty()) return e;
+                // depth-first numbering: an inner child comes after its parent,
+                // so no walk can cycle
+                if ((ref.y < 0 && ref.x <= w) || (ref.w < 0 && ref.z <= w)) return "a wide child before its parent";
+                wl.refs[(size_t)w] = ref;
             }
             return "";
         }
@@ -132,7 +169,7 @@ std::string check_array(int which, const char *raw, size_t bytes, const PersistH
                 std::string e = check_pair(((const int2 *)raw)[r], h.nwide, n);
                 if (!e.empty()) return e;
             }
-            return "";
+            return check_wide_tree(wl, (const int2 *)raw, h.nsets, depth);
         }
         case 8: {
             for (int64_t w = 0; w < h.nwide; w++) {
@@ -264,6 +301,7 @@ int ckd_tree_load(const char *path, void *stream, ckd_tree **out) {
     std::vector<char> host;
     int rc = CKD_OK;
     int32_t depth = 0;
+    WideLinks wl;
     for (size_t ai = 0; ai < as.size(); ai++) {
         Array &a = as[ai];
         if (a.bytes == 0) continue;
@@ -273,7 +311,7 @@ int ckd_tree_load(const char *path, void *stream, ckd_tree **out) {
             rc = CKD_ERR_ARG;
             break;
         }
-        const std::string why = check_array((int)ai, host.data(), a.bytes, h, depth);
+        const std::string why = check_array((int)ai, host.data(), a.bytes, h, depth, wl);
         if (!why.empty()) {
             set_error("%s: %s", path, why.c_str());
             rc = CKD_ERR_ARG;

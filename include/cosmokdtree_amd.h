@@ -305,6 +305,12 @@ int ckd_batch_plan(const ckd_tree *forest, const int64_t *qoff, const int32_t *k
    CKD_KNN_LANE (one walk per query; the default for sparser batches).
    Results never depend on it. */
 int ckd_batch_config(ckd_batch *plan, int32_t knn_kernel, int32_t queries_per_wave, int32_t stack_depth);
+/* Ball kernel of a plan: CKD_KNN_LANE (one walk per query, the default) or
+   CKD_KNN_STACK (the 64 queries of a wave share one walk over the set's
+   wide records, as ckd_ball_count/fill do; used when the forest has wide
+   records and its depth fits the stack).  Results never depend on it; row
+   order within a query may differ. */
+int ckd_batch_config_ball(ckd_batch *plan, int32_t ball_kernel);
 int ckd_batch_run(ckd_batch *plan, const void *queries, void *knn_dist, int64_t *knn_idx, int64_t *ball_off,
                   void *ball_dist, int64_t *ball_idx, void *stream);
 int ckd_batch_status(const ckd_batch *plan, const int64_t *ball_off, int64_t *ball_total, int32_t *overflow,

@@ -173,3 +173,37 @@ def test_batch_replan_invalidates_graph(ckd, workload):
         assert not none.has_ball and torch.count_nonzero(out["ball_off"]) == 0 and out["ball_idx"].numel() == 0
     finally:
         forest.free()
+
+
+@pytest.mark.parametrize("periodic,dtype,threads", [(True, np.float32, 256), (False, np.float64, 128),
+                                                    (True, np.float32, 64)])
+def test_batch_shared_ball_walk(ckd, workload, periodic, dtype, threads):
+    """Ball items on the shared walk (one walk per wave over each set's wide
+    records): every set's rows equal the oracle's, also when the first run
+    overflows the ball capacity and the plan is re-made."""
+    from cosmokdtree_amd import batch
+    sets, nq, ks, radii, qs = workload
+    sets = [s.astype(dtype) for s in sets]
+    qs = [q.astype(dtype) for q in qs]
+    box = [L] * 3 if periodic else None
+    forest = batch.build_forest(sets, leaf=16, boxsize=box)
+    print(f"\n[shared ball walk] periodic={periodic} {np.dtype(dtype).name} threads={threads}", flush=True)
+    try:
+        plan = batch.BatchPlan(forest, nq, ks, radii, sorted=True, threads=threads, ball_kernel="stack",
+                               ball_capacity=64)
+        out = plan.run(torch.from_numpy(np.concatenate(qs)).cuda())
+        assert plan.capacity > 64  # the first run overflowed and the plan was re-made
+        off = out["ball_off"].cpu().numpy()
+        bd, bi = out["ball_dist"].cpu().numpy(), out["ball_idx"].cpu().numpy()
+        lane = batch.BatchPlan(forest, nq, ks, radii, sorted=True, threads=threads, ball_kernel="lane")
+        lo = lane.run(torch.from_numpy(np.concatenate(qs)).cuda())
+        assert torch.equal(lo["ball_off"].cpu(), torch.from_numpy(off))
+        for r, (pts, q) in enumerate(zip(sets, qs)):
+            orc = OracleTree(pts, 16, None if box is None else np.full(3, L))
+            o_ref, bd_ref, bi_ref = orc.ball_csr(q, radii[r], False)
+            q0 = int(plan.qoff[r])
+            rows = off[q0:q0 + len(q) + 1]
+            assert_csr_equal(o_ref, bd_ref, bi_ref, rows - rows[0], bd[rows[0]:rows[-1]], bi[rows[0]:rows[-1]])
+            orc.close()
+    finally:
+        forest.free()
