@@ -193,8 +193,10 @@ __global__ __launch_bounds__(QT, 3) void k_knn_w(TreeView<T> tv, const T *querie
 // G lanes per query (1, 2, 4, ..., 64; 64 / G queries per wave): the grid
 // covers nq * G lanes.  GC = 1 compiles the one-lane-per-query walk with G
 // folded away (the group scans cost registers and time even when G is 1 at
-// run time); GC = 0 takes G from the argument.
-template <typename T, int D, bool PER, int KMAX, int SD, int GC>
+// run time); GC = 0 takes G from the argument.  ST = false compiles the
+// work counters away (calls without `stats`): live, their 64-bit sums held
+// ten registers and were spilled to scratch inside the walk.
+template <typename T, int D, bool PER, int KMAX, int SD, int GC, bool ST>
 __global__ __launch_bounds__(QT, sizeof(T) == 4 ? 4 : 3) void k_knn_s(TreeView<T> tv, int2 root, const T *queries, int64_t nq,
                                               const int64_t *nq_dev, int k, int sorted, const float *seed,
                                               const int32_t *order, const int32_t *leaf_of, T *dout,
@@ -228,7 +230,7 @@ __global__ __launch_bounds__(QT, sizeof(T) == 4 ? 4 : 3) void k_knn_s(TreeView<T
                                                           iout + qi * k, 0, c, buf_d, buf_s, threadIdx.x, QT,
                                                           stk[threadIdx.x / WAVE], found, G, sub, gbase, leaf);
     if (!done && sub == 0) send_retry(rt, t, qi, sr, found, k);
-    if (stats) {
+    if (ST && stats) {
         if (!active || sub != 0) c = Counters();
         flush_stats(c, active && sub == 0 && !nq_dev ? 1 : 0, stats);
     }
@@ -476,9 +478,15 @@ int launch_wave(const ckd_tree *t, const TreeView<T> &tv, int kind, int sd, int 
             const int2 root = make_int2(0, t->nnodes > 1 ? -1 : (int32_t)t->n);
             const int G = WAVE / qpw;
             const unsigned gs = (unsigned)ceil_div(nq * G, QT);
-#define CKD_KNN_S(SD_, GC_)                                                                                  \
-    k_knn_s<T, D, PER, KMAX, SD_, GC_><<<gs, QT, 0, s>>>(tv, root, q, nq, nq_dev, k, sorted, seed, order, leaf_of, \
-                                                         dist, idx, stats, rt, G)
+#define CKD_KNN_S(SD_, GC_)                                                                                    \
+    do {                                                                                                   \
+        if (stats)                                                                                         \
+            k_knn_s<T, D, PER, KMAX, SD_, GC_, true><<<gs, QT, 0, s>>>(tv, root, q, nq, nq_dev, k, sorted, seed,  \
+                                                                       order, leaf_of, dist, idx, stats, rt, G); \
+        else                                                                                               \
+            k_knn_s<T, D, PER, KMAX, SD_, GC_, false><<<gs, QT, 0, s>>>(tv, root, q, nq, nq_dev, k, sorted, seed, \
+                                                                        order, leaf_of, dist, idx, stats, rt, G); \
+    } while (0)
             if (sd <= 32) {
                 if (G == 1) CKD_KNN_S(32, 1);
                 else CKD_KNN_S(32, 0);

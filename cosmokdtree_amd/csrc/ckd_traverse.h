@@ -57,14 +57,19 @@ __device__ __forceinline__ T node_mindist(const TreeView<T> &tv, int32_t node, c
 
 // Insert (d, s), d < bd[KMAX-1], into the ascending list; the last entry
 // drops out and d goes after the entries equal to it.  Slot j becomes the
-// median of (old j-1, d, old j); the rows follow the same choice.
+// median of (old j-1, d, old j); the rows follow the same choice.  The row
+// of slot j is chosen in two steps (s or the old row, then the previous
+// slot's row): written as one nested conditional, the compiler turned it
+// into two divergent branches per slot inside the callers' `if`s, about
+// three times the instructions of the two selects.
 template <typename T, int KMAX>
 __device__ __forceinline__ void list_insert(T (&bd)[KMAX], int32_t (&bs)[KMAX], T d, int32_t s) {
 #pragma unroll
     for (int j = KMAX - 1; j > 0; --j) {
         const bool below = d < bd[j - 1];
         const bool here = d < bd[j];
-        bs[j] = below ? bs[j - 1] : (here ? s : bs[j]);
+        const int32_t keep = here ? s : bs[j];
+        bs[j] = below ? bs[j - 1] : keep;
         bd[j] = med3v(bd[j - 1], d, bd[j]);
     }
     bs[0] = d < bd[0] ? s : bs[0];
@@ -473,22 +478,20 @@ __device__ __forceinline__ void wave_scan_rows(const TreeView<T> &tv, int32_t pb
         for (int32_t p = pb; p < pe; p += PB) {
             const T bound = bd[KMAX - 1];
             int cnt = pend;
-            T *wd = buf_d + cnt * nthreads + tid;  // this lane's next free slot
-            int32_t *ws = buf_s + cnt * nthreads + tid;
             T x[PB][D];
             CKD_DASSERT(p >= 0 && p + PB <= tv.nrows + PTS_PAD);
             load_rows<T, D, PB>(tv.pts, p, x);
 #pragma unroll
             for (int j = 0; j < PB; j++) {
                 const T d = dist2<T, D, PER>(x[j], q, tv.L);
-                if (run && p + j < pe && d < bound) {
-                    CKD_DASSERT(cnt < CB);
-                    *wd = d;
-                    if constexpr (!NOIDX) *ws = p + j;
-                    wd += nthreads;
-                    ws += nthreads;
-                    cnt++;
-                }
+                // every lane writes its next free slot, taken only when the
+                // row is accepted: no branch per row.  The slot exists: a
+                // block starts with at most CB - PB slots filled.
+                CKD_DASSERT(cnt < CB);
+                const int slot = cnt * nthreads + tid;
+                buf_d[slot] = d;
+                if constexpr (!NOIDX) buf_s[slot] = p + j;
+                cnt += (run && p + j < pe && d < bound) ? 1 : 0;
             }
             pend = cnt;
             if (!__any(cnt > CB - PB)) continue;
