@@ -388,7 +388,7 @@ __global__ __launch_bounds__(BT) void k_chunk_countleft(const Chunk *chunks, con
 #pragma unroll
         for (int u = 0; u < RU; u++) {
             const int64_t p = base + u * WAVE + lane;
-            c += __popcll(__ballot(p < ch.end && goes_left(dc, x[u], p)));
+            c += __popcll(wave_vote(p < ch.end && goes_left(dc, x[u], p)));
         }
     }
     if (lane == 0) cleft[ci] = c;
@@ -437,7 +437,7 @@ __global__ __launch_bounds__(BT) void k_chunk_scatter(const Chunk *chunks, const
             for (int d = 1; d < D; d++)
                 if (d == dc.axis) xa = x[u][d];
             const bool f = in && goes_left(dc, xa, p);
-            const uint64_t m = __ballot(f);
+            const uint64_t m = wave_vote(f);
             const int rank = __popcll(m & below);
             const int nl = __popcll(m);
             const int cnt = ch.end - rb >= WAVE ? WAVE : (ch.end > rb ? (int)(ch.end - rb) : 0);

@@ -228,6 +228,23 @@ __device__ __forceinline__ int64_t load_idx(const void *idx, int idx64, int64_t 
 // ----------------------------------------------------- wave / block scans
 constexpr int WAVE = 64;
 
+// Wave votes on a bool: the mask of the active lanes where p holds, and
+// whether there is one.  HIP's __ballot / __any take an int; the bool ->
+// int -> bool round trip cost two vector instructions per vote.
+__device__ __forceinline__ uint64_t wave_vote(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+__device__ __forceinline__ bool wave_any(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0; }
+
+// The same straight from one compare (its lane mask; inactive lanes 0), for
+// votes combined with another lane mask by a scalar AND: a vote on
+// `run && a < b` was materialised as a 0/1 value and compared again.
+// Predicates as LLVM numbers them: 4 = ordered less than, 10 = unordered or
+// greater than, 38 = signed greater than.
+__device__ __forceinline__ uint64_t lanes_lt(float a, float b) { return __builtin_amdgcn_fcmpf(a, b, 4); }
+__device__ __forceinline__ uint64_t lanes_lt(double a, double b) { return __builtin_amdgcn_fcmp(a, b, 4); }
+__device__ __forceinline__ uint64_t lanes_ugt(float a, float b) { return __builtin_amdgcn_fcmpf(a, b, 10); }
+__device__ __forceinline__ uint64_t lanes_ugt(double a, double b) { return __builtin_amdgcn_fcmp(a, b, 10); }
+__device__ __forceinline__ uint64_t lanes_gt(int a, int b) { return __builtin_amdgcn_sicmp(a, b, 38); }
+
 // XCD-aware block order.  Workgroups are dealt to the 8 XCDs (each with its
 // own L2) round-robin by block id.  This maps the blocks an XCD receives
 // within each group of 8 * XCD_RUN consecutive ids to a run of XCD_RUN

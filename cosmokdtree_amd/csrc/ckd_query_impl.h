@@ -190,6 +190,10 @@ __global__ __launch_bounds__(QT, 3) void k_knn_w(TreeView<T> tv, const T *querie
 // Candidate buffering pays from k = 16 on (below, the short list update is
 // cheaper than the LDS round trip and the occupancy the buffer costs); 8
 // slots per lane in double precision keep the LDS within 4 blocks per CU.
+// At k <= 32 in float32 with one lane per query (91 registers) the LDS sets
+// the occupancy: 12 slots per lane (24 KB) and the 4 KB stack fit 5 blocks
+// per CU, 5 waves per SIMD; 16 slots fitted 4, and the self-query of the
+// clustered 2^24 box took 32.6 ms instead of 28.5.
 // G lanes per query (1, 2, 4, ..., 64; 64 / G queries per wave): the grid
 // covers nq * G lanes.  GC = 1 compiles the one-lane-per-query walk with G
 // folded away (the group scans cost registers and time even when G is 1 at
@@ -197,12 +201,13 @@ __global__ __launch_bounds__(QT, 3) void k_knn_w(TreeView<T> tv, const T *querie
 // work counters away (calls without `stats`): live, their 64-bit sums held
 // ten registers and were spilled to scratch inside the walk.
 template <typename T, int D, bool PER, int KMAX, int SD, int GC, bool ST>
-__global__ __launch_bounds__(QT, sizeof(T) == 4 ? 4 : 3) void k_knn_s(TreeView<T> tv, int2 root, const T *queries, int64_t nq,
+__global__ __launch_bounds__(QT, sizeof(T) == 4 ? (KMAX <= 32 && GC == 1 ? 5 : 4) : 3) void k_knn_s(TreeView<T> tv, int2 root, const T *queries, int64_t nq,
                                               const int64_t *nq_dev, int k, int sorted, const float *seed,
                                               const int32_t *order, const int32_t *leaf_of, T *dout,
                                               int64_t *iout, int64_t *stats, Retry rt, int Grt) {
     const int G = GC ? GC : Grt;
-    constexpr int CB = KMAX < 16 ? 0 : (sizeof(T) == 4 ? ((KMAX == 16 && GC == 1) ? 8 : 16) : 8);
+    constexpr int CB =
+        KMAX < 16 ? 0 : (sizeof(T) == 4 ? (GC == 1 ? (KMAX == 16 ? 8 : (KMAX == 32 ? 12 : 16)) : 16) : 8);
     __shared__ T buf_d[CB > 0 ? CB * QT : 1];
     __shared__ int32_t buf_s[CB > 0 ? CB * QT : 1];
     __shared__ StackEntry<T, D> stk[QT / WAVE][SD];
@@ -243,7 +248,9 @@ __global__ __launch_bounds__(QT, 3) void k_sph_s(TreeView<T> tv, int2 root, cons
                                               const int32_t *order, const int32_t *leaf_of, const T *mass,
                                               T *hout, T *rhoout) {
     constexpr int D = 3;
-    constexpr int CB = KMAX < 16 ? 0 : (sizeof(T) == 4 ? 16 : 8);
+    // 12 slots with rows, as k_knn_s; without rows 16 (4 bytes a slot: the
+    // LDS does not limit the occupancy, and fewer flushes pay)
+    constexpr int CB = KMAX < 16 ? 0 : (sizeof(T) == 4 ? (KMAX == 32 && !NOIDX ? 12 : 16) : 8);
     __shared__ T buf_d[CB > 0 ? CB * QT : 1];
     __shared__ int32_t buf_s[CB > 0 ? CB * QT : 1];
     __shared__ StackEntry<T, D> stk[QT / WAVE][SD];

@@ -62,18 +62,32 @@ __device__ __forceinline__ T node_mindist(const TreeView<T> &tv, int32_t node, c
 // slot's row): written as one nested conditional, the compiler turned it
 // into two divergent branches per slot inside the callers' `if`s, about
 // three times the instructions of the two selects.
+//
+// The slots are updated in chunks of 8 from the top; a chunk whose entries
+// are all at or below d for every lane taking part is left as it is, and so
+// is everything under it (the update leaves slot j unchanged when
+// d >= old j).  Candidates found late in a walk mostly enter near the end
+// of the list, so many insertions stop after one or two chunks.
 template <typename T, int KMAX>
 __device__ __forceinline__ void list_insert(T (&bd)[KMAX], int32_t (&bs)[KMAX], T d, int32_t s) {
+    constexpr int C = KMAX >= 16 ? 8 : KMAX;
 #pragma unroll
-    for (int j = KMAX - 1; j > 0; --j) {
-        const bool below = d < bd[j - 1];
-        const bool here = d < bd[j];
-        const int32_t keep = here ? s : bs[j];
-        bs[j] = below ? bs[j - 1] : keep;
-        bd[j] = med3v(bd[j - 1], d, bd[j]);
+    for (int c = KMAX / C - 1; c >= 0; --c) {
+        // slots below c*C + C change only for lanes with d < old bd[c*C + C - 1]
+        if (c < KMAX / C - 1 && !lanes_lt(d, bd[c * C + C - 1])) break;
+#pragma unroll
+        for (int j = c * C + C - 1; j >= (c == 0 ? 1 : c * C); --j) {
+            const bool below = d < bd[j - 1];
+            const bool here = d < bd[j];
+            const int32_t keep = here ? s : bs[j];
+            bs[j] = below ? bs[j - 1] : keep;
+            bd[j] = med3v(bd[j - 1], d, bd[j]);
+        }
+        if (c == 0) {
+            bs[0] = d < bd[0] ? s : bs[0];
+            bd[0] = minv(bd[0], d);
+        }
     }
-    bs[0] = d < bd[0] ? s : bs[0];
-    bd[0] = minv(bd[0], d);
 }
 
 // list_insert without the rows: slot j becomes med3(old j-1, d, old j), one
@@ -316,11 +330,11 @@ __device__ __forceinline__ bool knn_wave(const TreeView<T> &tv, const T (&q)[D],
                 } else {
                     const int32_t nearc = nr.child + (ref[nr.axis] < nr.split ? 0 : 1);
                     const int32_t farc = nearc == nr.child ? nr.child + 1 : nr.child;
-                    if (__any(run && node_mindist<T, D, PER>(tv, nearc, q) < bd[KMAX - 1])) {
+                    if (wave_any(run && node_mindist<T, D, PER>(tv, nearc, q) < bd[KMAX - 1])) {
                         cur = nearc;
                         continue;
                     }
-                    if (__any(run && node_mindist<T, D, PER>(tv, farc, q) < bd[KMAX - 1])) {
+                    if (wave_any(run && node_mindist<T, D, PER>(tv, farc, q) < bd[KMAX - 1])) {
                         cur = farc;
                         continue;
                     }
@@ -329,7 +343,7 @@ __device__ __forceinline__ bool knn_wave(const TreeView<T> &tv, const T (&q)[D],
                 c.climbed++;
                 const int32_t nearc = nr.child + (ref[nr.axis] < nr.split ? 0 : 1);
                 const int32_t farc = nearc == nr.child ? nr.child + 1 : nr.child;
-                if (from == nearc && __any(run && node_mindist<T, D, PER>(tv, farc, q) < bd[KMAX - 1])) {
+                if (from == nearc && wave_any(run && node_mindist<T, D, PER>(tv, farc, q) < bd[KMAX - 1])) {
                     cur = farc;
                     from = -1;
                     continue;
@@ -456,7 +470,7 @@ __device__ __forceinline__ void flush_candidates(T (&bd)[KMAX], int32_t (&bs)[KM
 // Leaf scan shared by the stack kernels: rows [pb, pe).  With CB > 0 a row
 // closer than the lane's k-th distance joins the lane's pending candidates;
 // they stay pending across leaves and are flushed only when some lane's
-// buffer could overflow on the next block of rows (and once when the walk
+// buffer could overflow on the next rows (and once when the walk
 // ends).  Flushing after every leaf instead made each leaf cost the rounds
 // of its busiest lane: most rounds found one or two lanes with a candidate.
 // The bound used meanwhile is the list's, which only shrinks at flushes, so
@@ -471,32 +485,29 @@ __device__ __forceinline__ void wave_scan_rows(const TreeView<T> &tv, int32_t pb
                                                uint32_t xm, int &pend) {
     constexpr int PB = sizeof(T) == 4 ? 8 : 4;  // rows loaded per batch (CB is a multiple)
     const bool run = run_q && (uint32_t)(pb - xs) >= xm;
-    if constexpr (CB > 0) {
-        // blocks of PB rows; after a block the buffers are flushed only if
-        // some lane could not take a whole next block
+    if constexpr (CB > 0 && NOIDX) {
+        // distances only (the flushes are cheap): blocks of PB rows, a lane's
+        // accepted rows written to its next slots, one flush test per block
 #pragma unroll 1
         for (int32_t p = pb; p < pe; p += PB) {
             const T bound = bd[KMAX - 1];
             int cnt = pend;
+            T *wd = buf_d + cnt * nthreads + tid;  // this lane's next free slot
             T x[PB][D];
             CKD_DASSERT(p >= 0 && p + PB <= tv.nrows + PTS_PAD);
             load_rows<T, D, PB>(tv.pts, p, x);
 #pragma unroll
             for (int j = 0; j < PB; j++) {
                 const T d = dist2<T, D, PER>(x[j], q, tv.L);
-                // every lane writes its next free slot, taken only when the
-                // row is accepted: no branch per row.  The slot exists: a
-                // block starts with at most CB - PB slots filled.
-                CKD_DASSERT(cnt < CB);
-                const int slot = cnt * nthreads + tid;
-                buf_d[slot] = d;
-                if constexpr (!NOIDX) buf_s[slot] = p + j;
-                cnt += (run && p + j < pe && d < bound) ? 1 : 0;
+                if (run && p + j < pe && d < bound) {
+                    CKD_DASSERT(cnt < CB);
+                    *wd = d;
+                    wd += nthreads;
+                    cnt++;
+                }
             }
             pend = cnt;
-            if (!__any(cnt > CB - PB)) continue;
-            // the flush, written out here: through flush_candidates the
-            // compiler kept more of the list in scratch
+            if (!lanes_gt(cnt, CB - PB)) continue;
             pend = 0;
             const int jmax = __reduce_max_sync(~0ull, cnt);
 #pragma unroll 1
@@ -504,10 +515,51 @@ __device__ __forceinline__ void wave_scan_rows(const TreeView<T> &tv, int32_t pb
                 if (j < cnt) {
                     const T dj = buf_d[j * nthreads + tid];
                     if (dj < bd[KMAX - 1]) {
-                        list_put<NOIDX, T, KMAX>(bd, bs, dj, NOIDX ? 0 : buf_s[j * nthreads + tid]);
+                        list_insert_d<T, KMAX>(bd, dj);
                         c.inserted++;
                     }
                 }
+            }
+        }
+    } else if constexpr (CB > 0) {
+        // groups of RS rows, loaded together; before each group the buffers
+        // are flushed only if some lane could not take all of its rows (a
+        // later flush than one test per 8 rows: each flush costs the rounds
+        // of its fullest lane, and lanes fill in bursts, at different
+        // times).  The rows are loaded after the test, so they are not held
+        // across a flush.
+        constexpr int RS = 4;
+        static_assert(CB > RS, "row groups");
+#pragma unroll 1
+        for (int32_t p = pb; p < pe; p += RS) {
+            if (lanes_gt(pend, CB - RS)) {
+                const int jmax = __reduce_max_sync(~0ull, pend);
+#pragma unroll 1
+                for (int j = 0; j < jmax; j++) {
+                    if (j < pend) {
+                        const T dj = buf_d[j * nthreads + tid];
+                        if (dj < bd[KMAX - 1]) {
+                            list_put<NOIDX, T, KMAX>(bd, bs, dj, NOIDX ? 0 : buf_s[j * nthreads + tid]);
+                            c.inserted++;
+                        }
+                    }
+                }
+                pend = 0;
+            }
+            const T bound = bd[KMAX - 1];
+            T x[RS][D];
+            CKD_DASSERT(p >= 0 && p + RS <= tv.nrows + PTS_PAD);
+            load_rows<T, D, RS>(tv.pts, p, x);
+#pragma unroll
+            for (int j = 0; j < RS; j++) {
+                const T d = dist2<T, D, PER>(x[j], q, tv.L);
+                // every lane writes its next free slot, taken only when the
+                // row is accepted: no branch per row
+                CKD_DASSERT(pend < CB);
+                const int slot = pend * nthreads + tid;
+                buf_d[slot] = d;
+                if constexpr (!NOIDX) buf_s[slot] = p + j;
+                pend += (run && p + j < pe && d < bound) ? 1 : 0;
             }
         }
     } else {
@@ -625,8 +677,9 @@ __device__ __forceinline__ bool stack_walk(const TreeView<T> &tv, const T (&q)[D
                                            int32_t (&bs)[KMAX], int32_t &a, int32_t &b, int &sp, Counters &c,
                                            T *buf_d, int32_t *buf_s, int tid, int nthreads, StackEntry<T, D> *stk,
                                            int G, int sub, int gbase, int32_t xs, uint32_t xm, int &pend) {
+    const uint64_t runm = wave_vote(run);  // votes below: lanes of a compare & runm
     for (;;) {
-        if (CHECK && !__any(run && !(safe2 >= bd[KMAX - 1]))) return false;
+        if (CHECK && !(lanes_ugt(bd[KMAX - 1], safe2) & runm)) return false;  // !(safe2 >= bound) nowhere
         c.visited++;
         bool descend = false;
         if (b >= 0) {
@@ -641,11 +694,12 @@ __device__ __forceinline__ bool stack_walk(const TreeView<T> &tv, const T (&q)[D
             load_wide<T, D>(tv.wide, a, b0, b1, r);
             const T bound = bd[KMAX - 1];
             const T m0 = mindist2<T, D, PER>(b0, q, tv.L), m1 = mindist2<T, D, PER>(b1, q, tv.L);
-            const bool n0 = __any(run && m0 < bound);
-            const bool n1 = __any(run && m1 < bound);
+            const bool n0 = (lanes_lt(m0, bound) & runm) != 0;
+            const bool n1 = (lanes_lt(m1, bound) & runm) != 0;
             if (n0 && n1) {
                 // both: enter first the child more lanes are nearer to, push the other
-                const bool first1 = __popcll(__ballot(run && m1 < m0)) > __popcll(__ballot(run && m0 < m1));
+                const bool first1 =
+                    (int)__popcll(lanes_lt(m1, m0) & runm) > (int)__popcll(lanes_lt(m0, m1) & runm);
                 CKD_DASSERT(sp < SD);
                 StackEntry<T, D> &e = stk[sp];
 #pragma unroll
@@ -677,7 +731,7 @@ __device__ __forceinline__ bool stack_walk(const TreeView<T> &tv, const T (&q)[D
             for (int d = 0; d < 2 * D; d++) bx[d] = e.box[d];
             const int32_t ea = __builtin_amdgcn_readfirstlane(e.a), eb = __builtin_amdgcn_readfirstlane(e.b);
             c.climbed++;
-            if (__any(run && mindist2<T, D, PER>(bx, q, tv.L) < bd[KMAX - 1])) {
+            if (lanes_lt(mindist2<T, D, PER>(bx, q, tv.L), bd[KMAX - 1]) & runm) {
                 a = ea;
                 b = eb;
                 found = true;
@@ -1044,7 +1098,7 @@ __device__ __forceinline__ int64_t ball_stack(const TreeView<T> &tv, int2 root, 
     int qn = 0;  // queued emits (wave-uniform)
     // take rows [s, s + n) whole for the lanes in `in`
     auto take = [&](bool in, int32_t s, int32_t n) {
-        const uint64_t m = __ballot(in);
+        const uint64_t m = wave_vote(in);
         if (in) {
             cnt += n;
             if (FILL) {
@@ -1072,7 +1126,7 @@ __device__ __forceinline__ int64_t ball_stack(const TreeView<T> &tv, int2 root, 
     }
     int32_t a = root.x, b = root.y;
     int sp = 0;
-    bool go = __any(run);
+    bool go = wave_any(run);
     while (go) {
         if (b >= 0) {  // a leaf some lanes only partly cover: test its rows
             constexpr int PB = sizeof(T) == 4 ? 8 : 4;  // rows per load batch (within the row padding)
@@ -1105,17 +1159,17 @@ __device__ __forceinline__ int64_t ball_stack(const TreeView<T> &tv, int2 root, 
             const bool reach1 = run && !(mindist2<T, D, PER>(b1, q, tv.L) > r2);
             const bool whole0 = reach0 && maxdist2<T, D>(b0, q) <= r2;
             const bool whole1 = reach1 && maxdist2<T, D>(b1, q) <= r2;
-            if (__any(whole0)) {
+            if (wave_any(whole0)) {
                 const int2 rr = r.y >= 0 ? make_int2(r.x, r.y) : tv.wrange[r.x];
                 take(whole0, rr.x, rr.y);
             }
-            if (__any(whole1)) {
+            if (wave_any(whole1)) {
                 const int2 rr = r.w >= 0 ? make_int2(r.z, r.w) : tv.wrange[r.z];
                 take(whole1, rr.x, rr.y);
             }
             if (FILL && qn > 0) drain_emits<T, D, PER>(tv, q, eq, qn, dout, iout, idx_base);
             const bool t0 = reach0 && !whole0, t1 = reach1 && !whole1;
-            const uint64_t m0 = __ballot(t0), m1 = __ballot(t1);
+            const uint64_t m0 = wave_vote(t0), m1 = wave_vote(t1);
             if (m0 && m1) {  // left first, the right one deferred with its lanes
                 CKD_DASSERT(sp < SD);
                 stk[sp].a = r.z;
