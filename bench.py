@@ -90,6 +90,30 @@ def cpu_baseline(n, q, k, cores):
     return out or None
 
 
+def knn_spot_checks(P, d, i, k, L, checks):
+    """Rows of a full-size kNN result checked by brute force on the device:
+    the row's points are distinct and sit at exactly the k smallest squared
+    minimum-image distances of all N points (computed in the reference's
+    operation order), and the reported distances are their correctly rounded
+    square roots.  Returns the number of exact rows."""
+    import torch
+
+    from cosmokdtree_amd.largebox import _min_image_d2
+    n = P.shape[0]
+    ok = 0
+    for j in range(checks):
+        r = (j * 2654435761) % n  # spread over the box
+        d2 = _min_image_d2(P, P[r], L)
+        ref_vals = torch.topk(d2, k, largest=False, sorted=True).values
+        row_i, row_d = i[r], d[r]
+        gd2 = d2[row_i]
+        good = torch.equal(torch.sort(gd2).values, ref_vals)
+        good = good and torch.equal(torch.sqrt(gd2.double()).float(), row_d)
+        good = good and len(set(row_i.tolist())) == k
+        ok += int(good)
+    return ok
+
+
 def _ev_ms(fn, reps=3):
     """Median HIP-event time of fn() on the current stream (after one warm-up)."""
     import torch
@@ -160,6 +184,8 @@ def main():
     ap.add_argument("--quantile", type=float, default=0.95)
     ap.add_argument("--cpu-n", type=int, default=1 << 24)
     ap.add_argument("--cpu-q", type=int, default=1 << 20)
+    ap.add_argument("--spot-checks", type=int, default=16,
+                    help="rows of the last kNN step checked by brute force against all N points")
     ap.add_argument("--table", action="store_true",
                     help="print the GPU rows of the calibration table (configs 1 and 2) as one JSON line")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -221,7 +247,7 @@ def main():
         del d, i
     torch.cuda.synchronize()
     stats = torch.zeros(6, dtype=torch.int64, device=dev)
-    d, i = tree.knn(P[:1 << 20], K, sorted=True, stats=stats)  # visit counts on a 1M sample (untimed)
+    d, i = tree.knn(P, K, sorted=True, stats=stats)  # work counts of the whole step (untimed)
     st = stats.cpu().numpy()
     del d, i
     torch.cuda.synchronize()
@@ -241,6 +267,8 @@ def main():
     s_c, s_i = 4, (8 if info.idx_bits == 64 else 4)
     bq = knn_bytes_per_query(3, s_c, s_i, K, N, info.nnodes, N)
     plain_qps, plain_ms = qps, ms_step
+    spot = knn_spot_checks(P, d, i, K, L, args.spot_checks)
+    log(f"brute-force spot checks of the last step's rows: {spot}/{args.spot_checks} exact")
     d_plain = d
     del i
 
@@ -295,10 +323,11 @@ def main():
     # queries) over its average launch duration
     achieved = bq * N / (kernel_ms / 1e3) / 1e9
     sd = 32 if info.depth <= 32 else 64
-    kernel_name = (f"k_knn_s<float, 3, true, {K}, {sd}, 1>" if info.depth <= 64
+    kernel_name = (f"k_knn_s<float, 3, true, {K}, {sd}, 1, false>" if info.depth <= 64
                    else f"k_knn_w<float, 3, true, {K}>")
 
     extra = dict(build_ms=build_ms, build_mpts_s=N / build_ms / 1e3, tree_nodes=int(info.nnodes),
+                 knn_spot_checks_exact=f"{spot}/{args.spot_checks}",
                  tree_depth=int(info.depth), nodes_visited_per_query=float(st[0] / max(st[3], 1)),
                  points_tested_per_query=float(st[1] / max(st[3], 1)),
                  insertions_per_query=float(st[4] / max(st[3], 1)),
@@ -422,6 +451,9 @@ def main():
         "extra": extra,
     }
     print(json.dumps(out), flush=True)
+    if spot != args.spot_checks:
+        log(f"kNN rows differ from brute force ({spot}/{args.spot_checks} exact): the value is not valid")
+        sys.exit(3)
 
 
 if __name__ == "__main__":
