@@ -6,16 +6,24 @@ halos and 30 % uniform (the clustered recipe of ``synthetic``, generated on
 the device).  Workload:
 
 * the full GPU build (leaf 16);
-* kNN, k = 16, of a random subset of 1e8 points (sorted rows);
+* kNN, k = 16, of a random subset of 1e8 points (sorted rows): the plain
+  search, and the learned-seed search (a network trained on an independent
+  2^30-point draw of the same recipe), whose rows must equal the plain ones;
+* the same queries through the batched engine (one-set forest of the same
+  points, one plan with the kNN and a ball search of radius 0.05: the
+  halos are dense enough that radius 0.5 returns ~5.6e10 rows);
 * box search with 1e6 cubes of side log-uniform in [0.5, 5] centred
-  uniformly in the box, so that many straddle the periodic faces.  The
-  reference's box search does not wrap (cosmokdtree.f90:1260) and neither
-  does this one: the part of a box outside [-L/2, L/2] holds no points.
+  uniformly in the box, taken modulo the box (``wrap``), so that the boxes
+  that straddle a periodic face find the points beyond it.  The reference's
+  box search does not wrap (cosmokdtree.f90:1260); this is the library's
+  periodic rule: the union of the wrapped pieces, each searched with the
+  reference's rules.
 
 Exactness at this scale is spot-checked on the device by brute force: the
 kNN rows of a few queries against all N minimum-image distances (computed
-in the reference's operation order, one rounding per operation), and the
-box rows of a few boxes against the inclusive per-point test.
+in the reference's operation order, one rounding per operation), the ball
+rows against the inclusive per-point test, and the box rows against the
+inclusive per-point test on the wrapped pieces.
 
     python -m cosmokdtree_amd.largebox [--n 1073741824] [--nq 1e8] [--nbox 1e6]
 
@@ -83,6 +91,32 @@ def _min_image_d2(P: torch.Tensor, q: torch.Tensor, L: float) -> torch.Tensor:
     return d2
 
 
+def wrapped_box_mask(P: torch.Tensor, b: torch.Tensor, L: float) -> torch.Tensor:
+    """Points inside box b (xmin, xmax, ymin, ...) taken modulo the periodic
+    box: on every axis inside one of the pieces [lo - jL, hi - jL] clipped to
+    [-L/2, L/2], j = m+1, m, m-1 with m = floor((lo + L/2) / L), the bounds
+    rounded to float32 as the search rounds them; the whole axis when
+    hi - lo >= L.  Inclusive, as the reference's leaf test."""
+    H = np.float32(L / 2)
+    inside = torch.ones(P.shape[0], dtype=torch.bool, device=P.device)
+    for ax in range(3):
+        lo, hi = np.float32(b[2 * ax].item()), np.float32(b[2 * ax + 1].item())
+        if not lo <= hi:
+            return torch.zeros_like(inside)
+        if np.float32(hi - lo) >= np.float32(L):
+            continue
+        m = math.floor((float(lo) + float(H)) / float(L))
+        x = P[:, ax]
+        on = torch.zeros_like(inside)
+        for j in (1, 0, -1):
+            sh = np.float32(m + j) * np.float32(L)
+            a_, b_ = max(np.float32(lo - sh), -H), min(np.float32(hi - sh), H)
+            if a_ <= b_:
+                on |= (x >= float(a_)) & (x <= float(b_))
+        inside &= on
+    return inside
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--n", type=int, default=1 << 30)
@@ -93,6 +127,8 @@ def main():
     ap.add_argument("--nbox", type=float, default=1e6)
     ap.add_argument("--checks", type=int, default=8)
     ap.add_argument("--seed", type=int, default=20261205)
+    ap.add_argument("--train-pairs", type=int, default=2_000_000)
+    ap.add_argument("--ball-r", type=float, default=0.05)
     args = ap.parse_args()
     from . import kdtree
 
@@ -132,19 +168,74 @@ def main():
     # kNN spot checks: the row's points are at exactly the k smallest squared
     # distances (so the row is a kNN set, ties aside), distinct, and the
     # reported distances are the correctly rounded square roots of theirs
-    ok_knn = 0
-    for j in range(args.checks):
-        r = j * (nq // args.checks)
-        d2 = _min_image_d2(P, Q[r], L)
+    def knn_row_exact(row_d, row_i, q):
+        d2 = _min_image_d2(P, q, L)
         ref_vals, _ = torch.topk(d2, k, largest=False, sorted=True)
-        row_i, row_d = i[r], d[r]
         gd2 = d2[row_i]
         ok = torch.equal(torch.sort(gd2).values, ref_vals)
         ok = ok and torch.equal(torch.sqrt(gd2.double()).float(), row_d)
-        ok = ok and len(set(row_i.tolist())) == k
-        ok_knn += int(ok)
-        del d2
+        return ok and len(set(row_i.tolist())) == k
+
+    rows = [j * (nq // args.checks) for j in range(args.checks)]
+    ok_knn = sum(int(knn_row_exact(d[r], i[r], Q[r])) for r in rows)
     log(f"kNN spot checks: {ok_knn}/{args.checks} rows exact")
+
+    # learned seeds: a network trained on an independent draw of the recipe
+    from . import learned
+    t0 = time.time()
+    T = clustered_periodic_device(n, L, args.halo, args.seed + 3)
+    ttree = kdtree._build(T, 16, [L] * 3, idx_bits=64, keep_points=True)
+    model = learned.fit(ttree, args.train_pairs, k=k, epochs=4, quantile=0.95, log=log)
+    ttree.free()
+    del T
+    table = learned.feature_table(learned.occupancy(tree))
+    torch.cuda.synchronize()
+    train_s = time.time() - t0
+    learned.seeded_knn(tree, model, Q[:1 << 20], k, table=table)  # warm-up
+    torch.cuda.synchronize()
+    e0.record()
+    d_l, i_l = learned.seeded_knn(tree, model, Q, k, table=table)
+    e1.record()
+    torch.cuda.synchronize()
+    seeded_ms = e0.elapsed_time(e1)
+    seeded_equal = bool(torch.equal(d_l, d))
+    del d_l, i_l
+    st = torch.zeros(6, dtype=torch.int64, device=dev)
+    learned.seeded_knn(tree, model, Q[:1 << 24], k, table=table, stats=st)  # fallback share (untimed)
+    st = st.cpu().numpy()
+    log(f"learned-seed kNN: {seeded_ms:.1f} ms ({nq / seeded_ms / 1e3:.1f} Mq/s, x{knn_ms / seeded_ms:.2f} vs plain), "
+        f"distances equal to the plain rows: {seeded_equal}, fallback {st[2] / max(st[3], 1):.4f}, "
+        f"training {train_s:.1f}s")
+
+    # batched engine: the same queries, kNN and ball search in one plan
+    from . import batch
+    r_ball = args.ball_r
+    forest = batch.build_forest((P, np.array([0, n])), leaf=16, boxsize=[L] * 3, idx_bits=64)
+    plan = batch.BatchPlan(forest, [nq], [k], [r_ball], sorted=True)
+    out = plan.run(Q)  # sizes the ball output
+    torch.cuda.synchronize()
+    e0.record()
+    out = plan.run(Q)
+    e1.record()
+    torch.cuda.synchronize()
+    batch_ms = e0.elapsed_time(e1)
+    d_b, i_b = plan.knn_rows(0)
+    batch_equal = bool(torch.equal(d_b, d))
+    boff, bdist, bidx = out["ball_off"], out["ball_dist"], out["ball_idx"]
+    ball_total = int(boff[-1])
+    ok_ball = 0
+    for r in rows:
+        d2 = _min_image_d2(P, Q[r], L)
+        want = set(torch.nonzero(d2 <= np.float32(r_ball) * np.float32(r_ball)).flatten().tolist())
+        a, b = int(boff[r]), int(boff[r + 1])
+        got_i = bidx[a:b]
+        good = set(got_i.tolist()) == want and len(want) == b - a
+        good = good and torch.equal(torch.sqrt(d2[got_i].double()).float(), bdist[a:b])
+        ok_ball += int(good)
+    log(f"batched plan (kNN k={k} + ball r={r_ball} of {nq} queries): {batch_ms:.1f} ms, kNN rows equal to the "
+        f"single-tree rows: {batch_equal}, {ball_total} ball results, ball spot checks {ok_ball}/{args.checks} exact")
+    del out, d_b, i_b, boff, bdist, bidx, plan
+    forest.free()
     del d, i, Q
 
     g.manual_seed(args.seed + 2)
@@ -154,32 +245,38 @@ def main():
     lo, hi = c - s[:, None] / 2, c + s[:, None] / 2
     boxes = torch.stack([lo[:, 0], hi[:, 0], lo[:, 1], hi[:, 1], lo[:, 2], hi[:, 2]], dim=1).float().contiguous()
     straddle = float(((lo < -L / 2) | (hi > L / 2)).any(dim=1).float().mean())
-    tree.box(boxes[:1000])  # warm-up
+    tree.box(boxes[:1000], wrap=True)  # warm-up
     torch.cuda.synchronize()
     e0.record()
-    off, bi = tree.box(boxes)
+    off, bi = tree.box(boxes, wrap=True)
     e1.record()
     torch.cuda.synchronize()
     box_ms = e0.elapsed_time(e1)
     total = int(off[-1])
-    log(f"box search of {nbox} boxes ({straddle:.3f} straddle a face): {box_ms:.1f} ms, {total} results")
+    log(f"box search of {nbox} boxes taken modulo the box ({straddle:.3f} straddle a face): {box_ms:.1f} ms, "
+        f"{total} results")
+    # spot checks: the boxes that straddle a face first
+    strad_rows = torch.nonzero(((lo < -L / 2) | (hi > L / 2)).any(dim=1)).flatten()[:args.checks // 2].tolist()
+    box_rows = strad_rows + [j * (nbox // args.checks) for j in range(args.checks - len(strad_rows))]
     ok_box = 0
-    for j in range(args.checks):
-        b = boxes[j * (nbox // args.checks)]
-        m = ((P[:, 0] >= b[0]) & (P[:, 0] <= b[1]) & (P[:, 1] >= b[2]) & (P[:, 1] <= b[3]) &
-             (P[:, 2] >= b[4]) & (P[:, 2] <= b[5]))
+    for r in box_rows:
+        m = wrapped_box_mask(P, boxes[r], L)
         want = set(torch.nonzero(m).flatten().tolist())
-        r = j * (nbox // args.checks)
-        got = set(bi[int(off[r]):int(off[r + 1])].tolist())
-        ok_box += int(want == got)
-    log(f"box spot checks: {ok_box}/{args.checks} rows exact")
+        got = bi[int(off[r]):int(off[r + 1])].tolist()
+        ok_box += int(want == set(got) and len(got) == len(want))
+    log(f"box spot checks: {ok_box}/{len(box_rows)} rows exact ({len(strad_rows)} straddling)")
     tree.free()
     print(json.dumps(dict(n=n, L=L, halo_frac=args.halo, idx_bits=int(info.idx_bits), nnodes=int(info.nnodes),
                           depth=int(info.depth), build_ms=build_ms, build_mpts_s=n / build_ms / 1e3,
                           knn_k=k, knn_queries=nq, knn_ms=knn_ms, knn_qps=nq / knn_ms * 1e3,
-                          knn_checks_exact=f"{ok_knn}/{args.checks}", boxes=nbox, box_ms=box_ms,
+                          knn_checks_exact=f"{ok_knn}/{args.checks}",
+                          seeded_knn_ms=seeded_ms, seeded_speedup_vs_plain=knn_ms / seeded_ms,
+                          seeded_equal_plain=seeded_equal, seeded_fallback=float(st[2] / max(st[3], 1)),
+                          batch_ms=batch_ms, batch_ball_r=r_ball, batch_ball_results=ball_total,
+                          batch_knn_equal_plain=batch_equal, batch_ball_checks_exact=f"{ok_ball}/{args.checks}",
+                          boxes=nbox, box_wrap=True, box_ms=box_ms,
                           boxes_per_s=nbox / box_ms * 1e3, box_results=total, box_straddle_frac=straddle,
-                          box_checks_exact=f"{ok_box}/{args.checks}")), flush=True)
+                          box_checks_exact=f"{ok_box}/{len(box_rows)}")), flush=True)
 
 
 if __name__ == "__main__":
