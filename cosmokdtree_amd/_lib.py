@@ -28,7 +28,7 @@ EXPORTS = (
     "ckd_build", "ckd_build_forest", "ckd_free", "ckd_get_info", "ckd_knn", "ckd_ball_count", "ckd_ball_fill",
     "ckd_box_count", "ckd_box_fill", "ckd_tree_dump", "ckd_last_error", "ckd_version", "ckd_knn_config",
     "ckd_knn_timing", "ckd_knn_learned", "ckd_mlp_table", "ckd_tree_save", "ckd_tree_load", "ckd_sph_density",
-    "ckd_release_memory",
+    "ckd_release_memory", "ckd_ball_search",
 )
 KNN_KERNELS = {"auto": -1, "lane": 0, "wave": 1, "stack": 2}
 
@@ -63,6 +63,7 @@ def _load():
     lib.ckd_tree_load.argtypes = [C.c_char_p, vp, C.POINTER(vp)]
     lib.ckd_ball_count.argtypes = [vp, vp, i64, dbl, vp, C.POINTER(i64), vp]
     lib.ckd_ball_fill.argtypes = [vp, vp, i64, dbl, i32, vp, vp, vp, vp]
+    lib.ckd_ball_search.argtypes = [vp, vp, i64, dbl, i32, vp, i64, vp, vp, C.POINTER(i64), vp]
     lib.ckd_box_count.argtypes = [vp, vp, i64, vp, C.POINTER(i64), vp]
     lib.ckd_box_fill.argtypes = [vp, vp, i64, vp, vp, vp]
     lib.ckd_box_count_periodic.argtypes = [vp, vp, i64, vp, C.POINTER(i64), vp]

@@ -18,6 +18,8 @@ int query_ball(const ckd_tree *t, const void *q, int64_t nq, double r, int64_t *
                void *dist, int64_t *idx, Scratch &sc, hipStream_t s);
 int query_box(const ckd_tree *t, const void *b, int64_t nb, int64_t *counts, const int64_t *off, int64_t *idx,
               int wrap, hipStream_t s);
+int query_ball_search(const ckd_tree *t, const void *q, int64_t nq, double r, int64_t *off, int64_t cap, void *dist,
+                      int64_t *idx, int64_t *total, int32_t *overflow, Scratch &sc, hipStream_t s);
 
 static thread_local char g_err[512] = "";
 
@@ -402,6 +404,55 @@ int ckd_ball_fill(const ckd_tree *t, const void *queries, int64_t nq, double rad
         rc = query_ball(t, queries, nq, radius, nullptr, offsets, dist, idx, sc, s);
     }
     if (rc || !sorted) return rc;
+    return t->prec == 4 ? segmented_sort<float>((float *)dist, idx, offsets, nq, 0, s)
+                        : segmented_sort<double>((double *)dist, idx, offsets, nq, 0, s);
+}
+
+int ckd_ball_search(const ckd_tree *t, const void *queries, int64_t nq, double radius, int32_t sorted,
+                    int64_t *offsets, int64_t capacity, void *dist, int64_t *idx, int64_t *total, void *stream) {
+    g_err[0] = 0;
+    if (!t || !offsets || !total) {
+        set_error("tree, offsets and total must not be NULL");
+        return CKD_ERR_ARG;
+    }
+    if (!(radius > 0)) {
+        set_error("Radius must be a positive number");
+        return CKD_ERR_ARG;
+    }
+    if (nq < 0 || (nq > 0 && !queries) || capacity < 0 || (capacity > 0 && (!dist || !idx))) {
+        set_error("invalid ball_search arguments");
+        return CKD_ERR_ARG;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    if (nq == 0) return counts_to_offsets(offsets, 0, total, s);
+    int64_t tot = 0;
+    int32_t unwritten = 0;  // rows not (all) written by the one walk
+    int rc;
+    // the one walk needs the stack walk's conditions (wide records, depth
+    // <= 64, 32-bit rows, ndim <= 3); otherwise, or with CKD_BALL_KERNEL=lane
+    // (one walk per query), count, then fill
+    const char *kenv = getenv("CKD_BALL_KERNEL");
+    const bool lane = kenv && kenv[0] == 'l';
+    if (!lane && t->ndim <= 3 && t->wide && t->depth <= 64 && t->n < INT32_MAX && nq < INT32_MAX) {
+        Scratch sc(s);
+        rc = query_ball_search(t, queries, nq, radius, offsets, capacity, dist, idx, &tot, &unwritten, sc, s);
+        if (rc) return rc;
+    } else {
+        {
+            Scratch sc(s);
+            rc = query_ball(t, queries, nq, radius, offsets + 1, nullptr, nullptr, nullptr, sc, s);
+        }
+        if (rc) return rc;
+        if ((rc = counts_to_offsets(offsets, nq, &tot, s))) return rc;
+        unwritten = 1;
+    }
+    *total = tot;
+    if (tot > capacity) return CKD_OK;  // the caller fills buffers of tot rows (ckd_ball_fill)
+    if (unwritten) {
+        Scratch sc(s);
+        if ((rc = query_ball(t, queries, nq, radius, nullptr, offsets, dist, idx, sc, s))) return rc;
+    }
+    if (!sorted || tot == 0) return CKD_OK;
     return t->prec == 4 ? segmented_sort<float>((float *)dist, idx, offsets, nq, 0, s)
                         : segmented_sort<double>((double *)dist, idx, offsets, nq, 0, s);
 }

@@ -245,6 +245,11 @@ __device__ __forceinline__ uint64_t lanes_ugt(float a, float b) { return __built
 __device__ __forceinline__ uint64_t lanes_ugt(double a, double b) { return __builtin_amdgcn_fcmp(a, b, 10); }
 __device__ __forceinline__ uint64_t lanes_gt(int a, int b) { return __builtin_amdgcn_sicmp(a, b, 38); }
 
+// Set bits of the lane mask m below this lane (v_mbcnt).
+__device__ __forceinline__ int lane_rank(uint64_t m) {
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
 // XCD-aware block order.  Workgroups are dealt to the 8 XCDs (each with its
 // own L2) round-robin by block id.  This maps the blocks an XCD receives
 // within each group of 8 * XCD_RUN consecutive ids to a run of XCD_RUN

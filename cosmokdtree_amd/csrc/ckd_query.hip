@@ -15,6 +15,10 @@ int query_ball_f32(const ckd_tree *, const void *, int64_t, double, int64_t *, c
                    Scratch &, hipStream_t);
 int query_ball_f64(const ckd_tree *, const void *, int64_t, double, int64_t *, const int64_t *, void *, int64_t *,
                    Scratch &, hipStream_t);
+int query_ball_search_f32(const ckd_tree *, const void *, int64_t, double, int64_t *, int64_t, void *, int64_t *,
+                          int64_t *, int32_t *, Scratch &, hipStream_t);
+int query_ball_search_f64(const ckd_tree *, const void *, int64_t, double, int64_t *, int64_t, void *, int64_t *,
+                          int64_t *, int32_t *, Scratch &, hipStream_t);
 int query_box_f32(const ckd_tree *, const void *, int64_t, int64_t *, const int64_t *, int64_t *, int, hipStream_t);
 int query_sph_f32(const ckd_tree *, const void *, int64_t, int, const void *, void *, void *, Scratch &, hipStream_t);
 int query_sph_f64(const ckd_tree *, const void *, int64_t, int, const void *, void *, void *, Scratch &, hipStream_t);
@@ -37,6 +41,12 @@ int query_ball(const ckd_tree *t, const void *q, int64_t nq, double r, int64_t *
                void *dist, int64_t *idx, Scratch &sc, hipStream_t s) {
     return t->prec == 4 ? query_ball_f32(t, q, nq, r, counts, off, dist, idx, sc, s)
                         : query_ball_f64(t, q, nq, r, counts, off, dist, idx, sc, s);
+}
+
+int query_ball_search(const ckd_tree *t, const void *q, int64_t nq, double r, int64_t *off, int64_t cap, void *dist,
+                      int64_t *idx, int64_t *total, int32_t *overflow, Scratch &sc, hipStream_t s) {
+    return t->prec == 4 ? query_ball_search_f32(t, q, nq, r, off, cap, dist, idx, total, overflow, sc, s)
+                        : query_ball_search_f64(t, q, nq, r, off, cap, dist, idx, total, overflow, sc, s);
 }
 
 int query_sph(const ckd_tree *t, const void *q, int64_t nq, int k, const void *mass, void *h, void *rho,

@@ -29,6 +29,11 @@ int query_ball_f64(const ckd_tree *t, const void *q, int64_t nq, double r, int64
     return ball_t<double>(t, q, nq, r, counts, off, dist, idx, sc, s);
 }
 
+int query_ball_search_f64(const ckd_tree *t, const void *q, int64_t nq, double r, int64_t *off, int64_t cap,
+                          void *dist, int64_t *idx, int64_t *total, int32_t *overflow, Scratch &sc, hipStream_t s) {
+    return ball_search_t<double>(t, q, nq, r, off, cap, dist, idx, total, overflow, sc, s);
+}
+
 int query_sph_f64(const ckd_tree *t, const void *q, int64_t nq, int k, const void *mass, void *h, void *rho,
                    Scratch &sc, hipStream_t s) {
     return t->periodic ? sph_launch<double, true>(t, (const double *)q, nq, k, (const double *)mass, (double *)h, (double *)rho, sc, s)

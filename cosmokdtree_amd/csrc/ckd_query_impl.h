@@ -402,6 +402,87 @@ __global__ __launch_bounds__(QT) void k_ball(TreeView<T> tv, const T *queries, i
     if (!FILL) counts[qi] = cnt;
 }
 
+// One-walk ball search (ball_stage): counts per query and staging records.
+template <typename T, int D, bool PER, int SD>
+__global__ __launch_bounds__(QT) void k_ball_stage(TreeView<T> tv, int2 root, int2 rootrows, const T *queries,
+                                                   int64_t nq, const int32_t *order, T r2, int64_t *counts,
+                                                   StageOut so) {
+    __shared__ BallEntry stk[QT / WAVE][SD];
+    const int64_t blk = xcd_block();
+    const int64_t t = blk * QT + threadIdx.x;
+    if (blk * QT + (threadIdx.x & ~(WAVE - 1)) >= nq) return;  // whole wave idle
+    const bool active = t < nq;
+    const int64_t qi = active ? (order ? (int64_t)order[t] : t) : 0;
+    T q[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) q[d] = active ? queries[qi * D + d] : T(0);
+    const int64_t c = ball_stage<T, D, PER, SD>(tv, root, tv.dbox, rootrows, q, active, r2, (int32_t)qi, so,
+                                                stk[threadIdx.x / WAVE]);
+    if (active) counts[qi] = c;
+}
+
+// The staging records to their CSR places: one block per taken chunk, a
+// thread per batch record (its accepted rows in row order); the subtrees
+// taken whole are spread over the block afterwards.  The distances are
+// computed again in the walk's formula (same bits), their square roots
+// taken here.  Nothing is written when the staging area overflowed or the
+// total exceeds `cap` (the caller then fills the rows with the second walk).
+template <typename T, int D, bool PER>
+__global__ __launch_bounds__(256) void k_ball_scatter(TreeView<T> tv, const T *queries, StageOut so, const int64_t *off,
+                                                      int64_t nq, int64_t cap, T *dout, int64_t *iout) {
+    const int32_t c = blockIdx.x;
+    if ((uint32_t)c >= *so.next_chunk || c >= so.nchunks || *so.overflow || off[nq] > cap) return;
+    __shared__ int4 ranges[STAGE_CHUNK];
+    __shared__ int nr;
+    if (threadIdx.x == 0) nr = 0;
+    __syncthreads();
+    const int fill = so.chunk_fill[c];
+    CKD_DASSERT(fill >= 0 && fill <= STAGE_CHUNK);
+#pragma unroll 1
+    for (int r = threadIdx.x; r < fill; r += blockDim.x) {
+        const int4 e = so.rec[c * STAGE_CHUNK + r];
+        if ((uint32_t)e.x & STAGE_RANGE) {
+            ranges[atomicAdd(&nr, 1)] = e;
+            continue;
+        }
+        CKD_DASSERT(e.x >= 0 && e.x < nq && e.z >= 0 && e.z < tv.nrows);
+        int64_t o = off[e.x] + e.y;
+        T q[D];
+#pragma unroll
+        for (int d = 0; d < D; d++) q[d] = queries[(int64_t)e.x * D + d];
+        uint32_t mk = (uint32_t)e.w;
+        while (mk) {
+            const int j = __builtin_ctz(mk);
+            mk &= mk - 1;
+            CKD_DASSERT(o < off[e.x + 1]);
+            T x[D];
+            load_pt<T, D>(tv.pts, e.z + j, x);
+            dout[o] = sqrt_rn<T>(dist2<T, D, PER>(x, q, tv.L));
+            iout[o] = load_idx(tv.idx, tv.idx64, e.z + j);
+            o++;
+        }
+    }
+    __syncthreads();
+#pragma unroll 1
+    for (int k = 0; k < nr; k++) {
+        const int4 e = ranges[k];
+        const int32_t qi = (int32_t)((uint32_t)e.x & ~STAGE_RANGE);
+        CKD_DASSERT(qi >= 0 && qi < nq && e.z >= 0 && e.z + e.w <= tv.nrows);
+        const int64_t o = off[qi] + e.y;
+        CKD_DASSERT(o >= off[qi] && o + e.w <= off[qi + 1]);
+        T q[D];
+#pragma unroll
+        for (int d = 0; d < D; d++) q[d] = queries[(int64_t)qi * D + d];
+#pragma unroll 1
+        for (int j = threadIdx.x; j < e.w; j += blockDim.x) {
+            T x[D];
+            load_pt<T, D>(tv.pts, e.z + j, x);
+            dout[o + j] = sqrt_rn<T>(dist2<T, D, PER>(x, q, tv.L));
+            iout[o + j] = load_idx(tv.idx, tv.idx64, e.z + j);
+        }
+    }
+}
+
 template <typename T, int D, bool FILL>
 __global__ __launch_bounds__(QT) void k_box(TreeView<T> tv, const T *boxes, int64_t nb, int64_t *counts,
                                             const int64_t *offsets, int64_t *iout) {
@@ -808,6 +889,76 @@ int ball_launch(const ckd_tree *t, const T *q, int64_t nq, T r2, int64_t *counts
     return CKD_OK;
 }
 
+// Whether the one-walk ball search runs on this tree and batch (the stack
+// walk's conditions); otherwise ckd_ball_search counts and fills in two.
+inline bool ball_one_walk_ok(const ckd_tree *t, int64_t nq) {
+    return t->ndim <= 3 && t->wide && t->depth <= 64 && t->n < INT32_MAX && nq < INT32_MAX;
+}
+
+// One-walk ball search: order the queries, walk once writing counts and
+// staging records, scan the counts into `off`, move the records to their
+// places.  The staging area holds cap records plus one chunk per wave (each
+// wave's last chunk is partly filled), so it overflows only when the total
+// exceeds cap.  *total and *overflow [host] are read back at the end (one
+// synchronisation of the stream).
+template <typename T, int D>
+int ball_search_launch(const ckd_tree *t, const T *q, int64_t nq, T r2, int64_t *off, int64_t cap, T *dist,
+                       int64_t *idx, int64_t *total, int32_t *overflow, Scratch &sc, hipStream_t s) {
+    if constexpr (D > 3) {
+        return CKD_ERR_ARG;
+    } else {
+        const TreeView<T> tv = view<T>(t);
+        const int32_t *order = nullptr;
+        if (nq >= ORDER_MIN_QUERIES && t->nnodes > 1) {
+            const int rc = order_queries<T, D>(t, tv, q, nq, &order, nullptr, sc, s);
+            if (rc) return rc;
+        }
+        const int64_t nwaves = ceil_div(nq, WAVE);
+        // slots are 32-bit: a larger capacity is staged up to the limit (the
+        // rows beyond it are then filled by the second walk)
+        const int64_t max_chunks = (INT32_MAX / STAGE_CHUNK) - 1;
+        int64_t nchunks = ceil_div(cap > 0 ? cap : 0, STAGE_CHUNK) + nwaves + 1;
+        if (nchunks > max_chunks) nchunks = max_chunks;
+        // scratch: counts (nq), chunk fills (nchunks), counter + flag, scan, records
+        const size_t b_cnt = ((size_t)nq * 8 + 255) & ~(size_t)255;
+        const size_t b_fill = ((size_t)nchunks * 4 + 255) & ~(size_t)255;
+        const size_t b_scan = (scan_tmp_bytes(nq) + 255) & ~(size_t)255;
+        const size_t b_rec = (size_t)nchunks * STAGE_CHUNK * sizeof(int4);
+        void *ws = nullptr;
+        int rc = sc.take(b_cnt + b_fill + 256 + b_scan + b_rec, &ws);
+        if (rc) return rc;
+        char *p = (char *)ws;
+        int64_t *counts = (int64_t *)p;
+        StageOut so;
+        so.chunk_fill = (int32_t *)(p + b_cnt);
+        so.next_chunk = (uint32_t *)(p + b_cnt + b_fill);
+        so.overflow = (int32_t *)(p + b_cnt + b_fill + 8);
+        void *scr = p + b_cnt + b_fill + 256;
+        so.rec = (int4 *)(p + b_cnt + b_fill + 256 + b_scan);
+        so.nchunks = (int32_t)nchunks;
+        k_fill_i64<<<1, 64, 0, s>>>((int64_t *)so.next_chunk, 2, 0);  // the counter and the flag: no memset node
+        const int2 root = make_int2(0, t->nnodes > 1 ? -1 : (int32_t)t->n);
+        const int2 rows = make_int2(0, (int32_t)t->n);
+        const unsigned g = (unsigned)ceil_div(nq, QT);
+        if (t->periodic) {
+            if (t->depth <= 32) k_ball_stage<T, D, true, 32><<<g, QT, 0, s>>>(tv, root, rows, q, nq, order, r2, counts, so);
+            else k_ball_stage<T, D, true, 64><<<g, QT, 0, s>>>(tv, root, rows, q, nq, order, r2, counts, so);
+        } else {
+            if (t->depth <= 32) k_ball_stage<T, D, false, 32><<<g, QT, 0, s>>>(tv, root, rows, q, nq, order, r2, counts, so);
+            else k_ball_stage<T, D, false, 64><<<g, QT, 0, s>>>(tv, root, rows, q, nq, order, r2, counts, so);
+        }
+        CKD_HIP(hipGetLastError());
+        if ((rc = scan_i64(counts, off, nq, true, scr, s))) return rc;
+        if (t->periodic) k_ball_scatter<T, D, true><<<(unsigned)nchunks, 256, 0, s>>>(tv, q, so, off, nq, cap, dist, idx);
+        else k_ball_scatter<T, D, false><<<(unsigned)nchunks, 256, 0, s>>>(tv, q, so, off, nq, cap, dist, idx);
+        CKD_HIP(hipGetLastError());
+        CKD_HIP(hipMemcpyAsync(total, off + nq, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        CKD_HIP(hipMemcpyAsync(overflow, so.overflow, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        CKD_HIP(hipStreamSynchronize(s));
+        return CKD_OK;
+    }
+}
+
 template <typename T, int D>
 int box_launch(const ckd_tree *t, const T *b, int64_t nb, int64_t *counts, const int64_t *off, int64_t *idx,
                int wrap, hipStream_t s) {
@@ -853,6 +1004,16 @@ int ball_t(const ckd_tree *t, const void *q, int64_t nq, double r, int64_t *coun
     const T rr = (T)r;   // radius is a real(prec) dummy (cosmokdtree.f90:1019)
     const T r2 = rr * rr;
     CKD_DIM_SWITCH(t->ndim, (ball_launch<T, D_>(t, (const T *)q, nq, r2, counts, off, (T *)dist, idx, sc, s)))
+    return CKD_ERR_ARG;
+}
+
+template <typename T>
+int ball_search_t(const ckd_tree *t, const void *q, int64_t nq, double r, int64_t *off, int64_t cap, void *dist,
+                  int64_t *idx, int64_t *total, int32_t *overflow, Scratch &sc, hipStream_t s) {
+    const T rr = (T)r;  // radius is a real(prec) dummy (cosmokdtree.f90:1019)
+    const T r2 = rr * rr;
+    CKD_DIM_SWITCH(t->ndim, (ball_search_launch<T, D_>(t, (const T *)q, nq, r2, off, cap, (T *)dist, idx, total,
+                                                       overflow, sc, s)))
     return CKD_ERR_ARG;
 }
 

@@ -166,19 +166,45 @@ class KDTree(int):
         return h, rho
 
     def ball(self, q, radius: float, sorted: bool = False):
-        """Ball search of device queries: CSR (offsets (nq+1), dist, idx) on the GPU."""
+        """Ball search of device queries: CSR (offsets (nq+1), dist, idx) on the GPU.
+
+        One walk (ckd_ball_search) into buffers sized from the rows per query
+        of this tree's previous ball search, scaled by (r / r_prev)^ndim;
+        when the rows do not fit, a second walk (ckd_ball_fill) writes them
+        into buffers of the exact size."""
         self._check_alive()
         q = self._queries(q)
         nq = q.shape[0]
+        radius = float(radius)
         off = torch.empty(nq + 1, dtype=torch.int64, device=self.device)
         total = C.c_int64(0)
-        check(lib.ckd_ball_count(self._ptr, _vp(q), nq, float(radius), _vp(off), C.byref(total), _stream()),
-              "ball_search")
-        dist = torch.empty(max(total.value, 1), dtype=self.torch_dtype, device=self.device)
-        idx = torch.empty(max(total.value, 1), dtype=torch.int64, device=self.device)
-        check(lib.ckd_ball_fill(self._ptr, _vp(q), nq, float(radius), int(bool(sorted)), _vp(off), _vp(dist),
-                                _vp(idx), _stream()), "ball_search")
-        return off, dist[:total.value], idx[:total.value]
+        cap = self._ball_capacity(nq, radius)
+        dist = torch.empty(max(cap, 1), dtype=self.torch_dtype, device=self.device)
+        idx = torch.empty(max(cap, 1), dtype=torch.int64, device=self.device)
+        check(lib.ckd_ball_search(self._ptr, _vp(q), nq, radius, int(bool(sorted)), _vp(off), cap, _vp(dist),
+                                  _vp(idx), C.byref(total), _stream()), "ball_search")
+        tot = total.value
+        if tot > cap:
+            dist = torch.empty(max(tot, 1), dtype=self.torch_dtype, device=self.device)
+            idx = torch.empty(max(tot, 1), dtype=torch.int64, device=self.device)
+            check(lib.ckd_ball_fill(self._ptr, _vp(q), nq, radius, int(bool(sorted)), _vp(off), _vp(dist),
+                                    _vp(idx), _stream()), "ball_search")
+        if nq > 0 and radius > 0:
+            self._ball_hint = (radius, tot / nq)
+        return off, dist[:tot], idx[:tot]
+
+    # rows per query assumed before a tree's first ball search, and the
+    # largest first guess (rows); later calls use the previous call's count
+    BALL_FIRST_ROWS_PER_QUERY = 64
+    BALL_FIRST_MAX_ROWS = 1 << 26
+
+    def _ball_capacity(self, nq: int, radius: float) -> int:
+        hint = getattr(self, "_ball_hint", None)
+        if hint is None:
+            return int(min(nq * self.BALL_FIRST_ROWS_PER_QUERY, self.BALL_FIRST_MAX_ROWS))
+        r0, per = hint
+        per = per * (radius / r0) ** self.ndim if r0 > 0 else per
+        return int(min(nq * per * 1.25 + 4096, 1 << 40))
 
     def box(self, boxes, wrap: bool = False):
         """Box search of device boxes (nb, 2*ndim): CSR (offsets, idx) on the GPU.

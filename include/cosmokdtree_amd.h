@@ -212,6 +212,21 @@ int ckd_ball_fill(const ckd_tree *tree, const void *queries, int64_t nq, double 
                   const int64_t *offsets, void *dist, int64_t *idx, void *stream);
 
 /*
+ * Ball search in one walk: the two calls above (pyball_search_call_1 and
+ * _2, cosmokdtree2py.f90:153-246) as one, where the rows fit.  Writes
+ * offsets [dev, nq+1] and *total [host] as ckd_ball_count does and, when
+ * *total <= capacity, the rows to dist [dev, capacity, prec] and idx
+ * [dev, capacity int64] as ckd_ball_fill does (sorted != 0: each row by
+ * (distance, index)).  When *total > capacity the rows are not written and
+ * the call still returns CKD_OK: ckd_ball_fill with buffers of *total rows
+ * and these offsets completes it (a second walk).  The walk stages its rows
+ * in library scratch of 16 bytes per row of capacity plus 8 KiB per 64
+ * queries; the call synchronises `stream`.
+ */
+int ckd_ball_search(const ckd_tree *tree, const void *queries, int64_t nq, double radius, int32_t sorted,
+                    int64_t *offsets, int64_t capacity, void *dist, int64_t *idx, int64_t *total, void *stream);
+
+/*
  * Box search over nb boxes [dev, (nb, 2*ndim) = (xmin, xmax, ymin, ...)],
  * first pass (CSR offsets + host total).  Replaces pybox_search_call_1
  * (cosmokdtree2py.f90:252-287) -> box_search (cosmokdtree.f90:1220-1253),
