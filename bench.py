@@ -368,6 +368,11 @@ def main():
         tb = time.perf_counter() - t0
         extra["ball_r0.2_1M_qps"] = (1 << 20) / tb
         extra["ball_r0.2_mean_count"] = float(off[-1].item()) / (1 << 20)
+        # algorithmic bytes per query: query read, offset, c_q rows of (f32
+        # distance + int64 index); the whole call (one walk) over HBM peak
+        bq_ball = 3 * 4 + 8 + extra["ball_r0.2_mean_count"] * (4 + 8)
+        extra["ball_bytes_per_query"] = bq_ball
+        extra["ball_roofline_frac"] = bq_ball * (1 << 20) / tb / 1e9 / HBM_PEAK_GBS
         log(f"ball r=0.2 on 1M: {tb * 1e3:.1f} ms, mean count {extra['ball_r0.2_mean_count']:.1f}")
         del off, bd, bi, sub
         # float64 build and kNN
