@@ -35,6 +35,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "kNN k=32 queries/s on 16.7M periodic pts, learned-seed vs plain; build Mpts/s"
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0  # MI355X dense bf16 MFMA
 
 
 def log(msg):
@@ -313,6 +314,18 @@ def main():
         log(f"learned seed: {seeded_ms:.2f} ms/step ({N / seeded_ms / 1e3:.1f} Mq/s), "
             f"x{plain_ms / seeded_ms:.2f} vs plain, exact={exact}, fallback {learned_info['fallback_rate']:.4f}")
         del d, i
+        # the network's forward pass alone over the step's features (bf16 MFMA
+        # tiles): 2*(32*64 + 64*64 + 64) flops per query against the dense
+        # bf16 peak
+        X = learned.features(tree, P, table)
+        fwd_ms = _ev_ms(lambda: model.seeds(X))
+        del X
+        flop_q = 2 * (32 * 64 + 64 * 64 + 64)
+        learned_info["mlp_forward_ms"] = fwd_ms
+        learned_info["mlp_forward_tflops"] = flop_q * N / (fwd_ms / 1e3) / 1e12
+        learned_info["mlp_forward_mfma_frac"] = learned_info["mlp_forward_tflops"] / BF16_PEAK_TFLOPS
+        log(f"MLP forward over {N} queries: {fwd_ms:.3f} ms, {learned_info['mlp_forward_tflops']:.1f} TFLOP/s "
+            f"({100 * learned_info['mlp_forward_mfma_frac']:.2f} % of the bf16 dense peak)")
     del d_plain
     kernel_ms = plain_kernel_ms
     use_learned = bool(learned_info and learned_info["distances_identical"] and learned_info["seeded_qps"] > qps)
