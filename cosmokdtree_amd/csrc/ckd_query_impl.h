@@ -15,13 +15,14 @@ constexpr int QT = 256;  // threads per block
 constexpr int64_t ORDER_MIN_QUERIES = 4096;  // below this, queries run in input order
 
 // Scratch bytes of a kNN call: the leaf ordering of the queries (leaf_of,
-// rank, order as int32 x nq; leaf counts and offsets as int64 x nnodes+1;
-// scan scratch), taken when the batch is large enough to be ordered, and
+// rank, order as int32 x nq; a flag set when the queries are not the
+// tree's points; leaf counts and offsets as int64 x nnodes+1; scan scratch), taken
+// when the batch is large enough to be ordered, and
 // the re-search lists of a seeded call (flag and offsets int64 x nq+1,
 // two lists int32 x nq, next seeds float x nq, a counter, scan scratch).
 inline size_t order_scratch_bytes(const ckd_tree *t, int64_t nq) {
     const size_t a = 3 * sizeof(int32_t) * (size_t)nq, b = sizeof(int64_t) * (size_t)(t->nnodes + 1);
-    return a + 2 * b + scan_tmp_bytes(t->nnodes) + 256;
+    return a + 2 * b + scan_tmp_bytes(t->nnodes) + 512;
 }
 inline bool orders_queries(const ckd_tree *t, int64_t nq) {
     return nq >= ORDER_MIN_QUERIES && nq < INT32_MAX && t->nnodes > 1;
@@ -33,10 +34,13 @@ inline size_t retry_scratch_bytes(int64_t nq) {
 
 // Query ordering: each query's leaf, and its rank among the queries of
 // that leaf; a counting sort then lists the queries leaf by leaf so a wave
-// works on neighbouring queries.
+// works on neighbouring queries.  `differ` is the flag k_self_order left
+// when the queries may be the tree's own points: 0 means the ordering is
+// already made and these passes return at once (nullptr: they always run).
 template <typename T, int D>
 __global__ __launch_bounds__(QT) void k_locate(TreeView<T> tv, const T *queries, int64_t nq, int32_t *leaf_of,
-                                               int32_t *rank, int64_t *leaf_cnt) {
+                                               int32_t *rank, int64_t *leaf_cnt, const int32_t *differ) {
+    if (differ && *differ == 0) return;
     const int64_t qi = (int64_t)blockIdx.x * QT + threadIdx.x;
     if (qi >= nq) return;
     T q[D];
@@ -48,28 +52,75 @@ __global__ __launch_bounds__(QT) void k_locate(TreeView<T> tv, const T *queries,
 }
 
 // k_locate plus the learned features of each query (3-D periodic trees),
-// computed while the leaf is at hand.
+// computed while the leaf is at hand; with the ordering already made the
+// features are still written, from the leaves k_self_order found.
 template <typename T>
 __global__ __launch_bounds__(QT) void k_locate_feat(TreeView<T> tv, const T *queries, int64_t nq, int32_t *leaf_of,
                                                     int32_t *rank, int64_t *leaf_cnt, double L0, double L1,
-                                                    double L2, int G, const uint16_t *tab, uint16_t *X) {
+                                                    double L2, int G, const uint16_t *tab, uint16_t *X,
+                                                    const int32_t *differ) {
+    const bool made = differ && *differ == 0;
     const int64_t qi = (int64_t)blockIdx.x * QT + threadIdx.x;
     if (qi >= nq) return;
     T q[3];
 #pragma unroll
     for (int d = 0; d < 3; d++) q[d] = queries[qi * 3 + d];
-    const int32_t leaf = find_leaf<T, 3>(tv, q, 0);
-    leaf_of[qi] = leaf;
-    rank[qi] = (int32_t)atomicAdd((unsigned long long *)&leaf_cnt[leaf], 1ull);
+    int32_t leaf;
+    if (made) {
+        leaf = leaf_of[qi];
+    } else {
+        leaf = find_leaf<T, 3>(tv, q, 0);
+        leaf_of[qi] = leaf;
+        rank[qi] = (int32_t)atomicAdd((unsigned long long *)&leaf_cnt[leaf], 1ull);
+    }
     const double Ls[3] = {L0, L1, L2};
     query_features<T>(tv, q, leaf, Ls, G, tab, X + qi * 32);
 }
 
 __global__ void k_order(const int32_t *leaf_of, const int32_t *rank, const int64_t *leaf_off, int64_t nq,
-                        int32_t *order) {
+                        int32_t *order, const int32_t *differ) {
+    if (differ && *differ == 0) return;
     const int64_t qi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (qi >= nq) return;
     order[leaf_off[leaf_of[qi]] + rank[qi]] = (int32_t)qi;
+}
+
+// Self-query ordering (nq == n, one point set).  The tree stores its rows
+// leaf by leaf, so taking query idx[r] at position r lists the queries leaf
+// by leaf with no locate pass and no sort, and a query's leaf is the leaf
+// holding its row -- if query idx[r] is row r's point.  Two passes: each
+// leaf writes its id over its rows (k_row_leaf), then one thread per row
+// writes the order and the query's leaf, and every 16th row compares its
+// query's coordinates with the row's; a wave that finds one differing sets
+// *differ, which makes k_locate and k_order redo the ordering from the
+// coordinates.  Results never depend on the ordering (it only groups the
+// walks and picks the rows that prefill the lists), so the check only has
+// to tell the tree's own points from other points, and a sample does: a
+// batch that differs in a few rows is still ordered well by the rows.
+template <typename T>
+__global__ __launch_bounds__(QT) void k_row_leaf(TreeView<T> tv, int32_t *row_leaf) {
+    const int64_t v = (int64_t)blockIdx.x * QT + threadIdx.x;
+    if (v >= tv.nnodes || tv.nodes[v].axis >= 0) return;
+    const Range r = tv.range[v];
+    for (int64_t p = r.start; p < r.start + r.count; p++) row_leaf[p] = (int32_t)v;
+}
+
+template <typename T, int D>
+__global__ __launch_bounds__(QT) void k_self_order(TreeView<T> tv, const T *queries, const int32_t *row_leaf,
+                                                   int32_t *leaf_of, int32_t *order, int32_t *differ) {
+    constexpr int PS = pstride(D);
+    const int64_t p = (int64_t)blockIdx.x * QT + threadIdx.x;
+    bool same = true;
+    if (p < tv.nrows) {
+        const int64_t qi = tv.idx64 ? ((const int64_t *)tv.idx)[p] : (int64_t)((const int32_t *)tv.idx)[p];
+        if ((p & 15) == 0) {
+#pragma unroll
+            for (int d = 0; d < D; d++) same = same && queries[qi * D + d] == tv.pts[p * PS + d];
+        }
+        order[p] = (int32_t)qi;
+        leaf_of[qi] = row_leaf[p];
+    }
+    if (wave_any(!same) && (threadIdx.x & (WAVE - 1)) == 0) *differ = 1;
 }
 
 __global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
@@ -685,6 +736,52 @@ inline int learned_buffers(Scratch &sc, int64_t nq, uint16_t **X, float **seeds)
     return CKD_OK;
 }
 
+// The leaf ordering of a batch (orders_queries) in scratch: order (the
+// query at each position) and leaf_of (each query's leaf).  With X (learned
+// seeds, 3-D trees) the same pass writes each query's features.  A batch as
+// large as the tree's one point set first tries the tree's own row order
+// (k_self_order); the locate pass and the counting sort then do their work
+// only if the queries are not the tree's points, as a device count decides,
+// so the host does not wait.
+template <typename T, int D>
+int order_queries(const ckd_tree *t, const TreeView<T> &tv, const T *q, int64_t nq, const int32_t **order,
+                  const int32_t **leaf_of, Scratch &sc, hipStream_t s, const Learned *ln = nullptr,
+                  uint16_t *X = nullptr) {
+    const unsigned g = (unsigned)ceil_div(nq, QT);
+    const size_t a = 3 * sizeof(int32_t) * (size_t)nq;
+    void *ws = nullptr;
+    int rc = sc.take(order_scratch_bytes(t, nq), &ws);
+    if (rc) return rc;
+    int32_t *lo = (int32_t *)ws, *rk = lo + nq, *ord = rk + nq;
+    int32_t *differ = (int32_t *)((char *)ws + ((a + 255) & ~(size_t)255));
+    int64_t *cnt = (int64_t *)((char *)differ + 256);
+    int64_t *off = cnt + t->nnodes + 1;
+    void *scr = off + t->nnodes + 1;
+    const int32_t *self = nullptr;
+    if (nq == t->n && t->nsets == 1) {
+        k_fill_i32<<<1, 64, 0, s>>>(differ, 1, 0);
+        k_row_leaf<T><<<(unsigned)ceil_div(t->nnodes, QT), QT, 0, s>>>(tv, rk);
+        k_self_order<T, D><<<g, QT, 0, s>>>(tv, q, rk, lo, ord, differ);
+        self = differ;
+    }
+    k_fill_i64<<<(unsigned)ceil_div(t->nnodes, 256), 256, 0, s>>>(cnt, t->nnodes, 0);
+    bool feats = false;
+    if constexpr (D == 3) {
+        if (X) {
+            k_locate_feat<T><<<g, QT, 0, s>>>(tv, q, nq, lo, rk, cnt, t->L[0], t->L[1], t->L[2], ln->G, ln->table,
+                                              X, self);
+            feats = true;
+        }
+    }
+    if (!feats) k_locate<T, D><<<g, QT, 0, s>>>(tv, q, nq, lo, rk, cnt, self);
+    if ((rc = scan_i64(cnt, off, t->nnodes, false, scr, s))) return rc;
+    k_order<<<(unsigned)ceil_div(nq, 256), 256, 0, s>>>(lo, rk, off, nq, ord, self);
+    CKD_HIP(hipGetLastError());
+    *order = ord;
+    if (leaf_of) *leaf_of = lo;
+    return CKD_OK;
+}
+
 template <typename T, int D, bool PER>
 int knn_launch(const ckd_tree *t, const T *q, int64_t nq, int k, int sorted, const float *seed, const Learned *ln,
                T *dist, int64_t *idx, int64_t *stats, Scratch &sc, hipStream_t s) {
@@ -718,31 +815,15 @@ int knn_launch(const ckd_tree *t, const T *q, int64_t nq, int k, int sorted, con
         float *seeds = nullptr;
         bool feats_done = false;
         if (orders_queries(t, nq)) {
-            // scratch: leaf_of, rank, order (int32 x nq), leaf counts and
-            // offsets (int64 x nnodes+1), scan scratch
-            const size_t a = 3 * sizeof(int32_t) * (size_t)nq, b = sizeof(int64_t) * (size_t)(t->nnodes + 1);
-            void *ws = nullptr;
-            int rc = sc.take(order_scratch_bytes(t, nq), &ws);
-            if (rc) return rc;
-            int32_t *lo = (int32_t *)ws, *rk = lo + nq, *ord = rk + nq;
-            int64_t *cnt = (int64_t *)((char *)ws + ((a + 255) & ~(size_t)255));
-            int64_t *off = cnt + t->nnodes + 1;
-            void *scr = off + t->nnodes + 1;
-            k_fill_i64<<<(unsigned)ceil_div(t->nnodes, 256), 256, 0, s>>>(cnt, t->nnodes, 0);
+            int rc;
             if constexpr (D == 3) {
                 if (ln) {  // learned: the features come with the leaves
                     if ((rc = learned_buffers(sc, nq, &X, &seeds))) return rc;
-                    k_locate_feat<T><<<g, QT, 0, s>>>(tv, q, nq, lo, rk, cnt, t->L[0], t->L[1], t->L[2], ln->G,
-                                                      ln->table, X);
                     feats_done = true;
                 }
             }
-            if (!feats_done) k_locate<T, D><<<g, QT, 0, s>>>(tv, q, nq, lo, rk, cnt);
-            if ((rc = scan_i64(cnt, off, t->nnodes, false, scr, s))) return rc;
-            k_order<<<(unsigned)ceil_div(nq, 256), 256, 0, s>>>(lo, rk, off, nq, ord);
-            CKD_HIP(hipGetLastError());
-            order = ord;
-            leaf_of = lo;
+            if ((rc = order_queries<T, D>(t, tv, q, nq, &order, &leaf_of, sc, s, ln, feats_done ? X : nullptr)))
+                return rc;
         }
         if (ln) {
             // learned seeds: the features (unless the ordering pass made
@@ -767,29 +848,6 @@ int knn_launch(const ckd_tree *t, const T *q, int64_t nq, int k, int sorted, con
     k_knn_heap<T, D, PER><<<g, QT, 0, s>>>(tv, q, nq, k, dist, idx);
     CKD_HIP(hipGetLastError());
     if (sorted) return segmented_sort<T>(dist, idx, nullptr, nq, k, s);
-    return CKD_OK;
-}
-
-// Leaf order of the queries (k_locate + counting sort) into workspace slot 0.
-template <typename T, int D>
-int order_queries(const ckd_tree *t, const TreeView<T> &tv, const T *q, int64_t nq, const int32_t **order,
-                  const int32_t **leaf_of, Scratch &sc, hipStream_t s) {
-    const unsigned g = (unsigned)ceil_div(nq, QT);
-    const size_t a = 3 * sizeof(int32_t) * (size_t)nq, b = sizeof(int64_t) * (size_t)(t->nnodes + 1);
-    void *ws = nullptr;
-    int rc = sc.take(order_scratch_bytes(t, nq), &ws);
-    if (rc) return rc;
-    int32_t *lo = (int32_t *)ws, *rk = lo + nq, *ord = rk + nq;
-    int64_t *cnt = (int64_t *)((char *)ws + ((a + 255) & ~(size_t)255));
-    int64_t *off = cnt + t->nnodes + 1;
-    void *scr = off + t->nnodes + 1;
-    k_fill_i64<<<(unsigned)ceil_div(t->nnodes, 256), 256, 0, s>>>(cnt, t->nnodes, 0);
-    k_locate<T, D><<<g, QT, 0, s>>>(tv, q, nq, lo, rk, cnt);
-    if ((rc = scan_i64(cnt, off, t->nnodes, false, scr, s))) return rc;
-    k_order<<<(unsigned)ceil_div(nq, 256), 256, 0, s>>>(lo, rk, off, nq, ord);
-    CKD_HIP(hipGetLastError());
-    *order = ord;
-    if (leaf_of) *leaf_of = lo;
     return CKD_OK;
 }
 

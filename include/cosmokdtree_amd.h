@@ -110,6 +110,12 @@ int ckd_get_info(const ckd_tree *tree, ckd_tree_info *info);
  * into a result list, parent steps climbed or deferred subtrees popped}.
  * The wave kernels count per wave: nodes visited and pops are wave totals
  * summed over the lanes.
+ * A large batch is walked in the order of the leaves its queries fall in.
+ * When nq equals the tree's point count, the library first checks on the
+ * device whether query idx[r] is the tree's row r for every row (the
+ * self-query of the tree's own points, in any array); if so the tree's row
+ * order is that ordering and the locate pass is skipped.  Results do not
+ * depend on which ordering ran.
  */
 int ckd_knn(const ckd_tree *tree, const void *queries, int64_t nq, int32_t k, int32_t sorted, void *dist,
             int64_t *idx, const float *seed_r, int64_t *stats, void *stream);

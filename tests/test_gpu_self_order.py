@@ -1,0 +1,74 @@
+"""Queries that are the tree's own points (nq == n): the batch is taken in
+the tree's row order, with each query's leaf read off its row, instead of
+the locate pass and the counting sort.  Whether a batch is the tree's points
+is decided on the device from every 16th row; a batch that is not (another
+order, other points of the same count) is ordered from its coordinates, and
+one that differs in a single row may go either way.  Every variant must give
+the oracle's rows: the ordering only groups the walks and picks the rows
+that prefill the lists."""
+import numpy as np
+import pytest
+import torch
+
+from oracle.oracle import OracleTree
+from tests.parity import assert_csr_equal, assert_knn_rows
+
+pytestmark = pytest.mark.gpu
+
+
+def _points(kind):
+    rng = np.random.default_rng(5)
+    if kind == "clustered_f32":
+        from cosmokdtree_amd.synthetic import clustered_periodic
+        return clustered_periodic(30000, 100.0, 0.5, seed=5, dtype=np.float32), np.full(3, 100.0)
+    return rng.random((20000, 3)).astype(np.float64), None
+
+
+def _variants(pts):
+    rng = np.random.default_rng(9)
+    one = pts.copy()
+    one[len(pts) // 2] = pts[0]  # a duplicate of row 0 in place of one row
+    other = pts[rng.integers(0, len(pts), len(pts))].astype(np.float64)
+    other = (other + (rng.random(other.shape) - 0.5) * 0.01).astype(pts.dtype)
+    return {"same": pts.copy(), "permuted": pts[rng.permutation(len(pts))], "one_row_changed": one,
+            "other_points": other}
+
+
+@pytest.fixture(scope="module", params=["clustered_f32", "uniform_f64"])
+def built(request, ckd):
+    pts, L = _points(request.param)
+    print(f"\n[self-order {request.param}] n={len(pts)}", flush=True)
+    tree = ckd.kdtree._build(pts, 16, L)
+    orc = OracleTree(pts, 16, L)
+    yield dict(pts=pts, L=L, tree=tree, orc=orc)
+    tree.free()
+    orc.close()
+
+
+@pytest.mark.parametrize("variant", ["same", "permuted", "one_row_changed", "other_points"])
+@pytest.mark.parametrize("k", [8, 32])
+def test_knn_self_order(built, variant, k):
+    q = _variants(built["pts"])[variant]
+    d_ref, i_ref = built["orc"].knn(q, k, True)
+    d_got, i_got = built["tree"].knn(torch.from_numpy(q), k, sorted=True)
+    assert_knn_rows(built["pts"], q, k, d_ref, i_ref, d_got.cpu().numpy(), i_got.cpu().numpy(), L=built["L"])
+
+
+def test_knn_self_order_same_tensor(built):
+    """The very array the tree was built from, on the device, twice in a row
+    (the second call reuses the scratch whose count the first one left)."""
+    P = torch.from_numpy(built["pts"]).cuda()
+    d_ref, i_ref = built["orc"].knn(built["pts"], 16, True)
+    for _ in range(2):
+        d_got, i_got = built["tree"].knn(P, 16, sorted=True)
+        assert_knn_rows(built["pts"], built["pts"], 16, d_ref, i_ref, d_got.cpu().numpy(), i_got.cpu().numpy(),
+                        L=built["L"])
+
+
+@pytest.mark.parametrize("variant", ["same", "permuted"])
+def test_ball_self_order(built, variant):
+    q = _variants(built["pts"])[variant]
+    r = 1.0 if built["L"] is not None else 0.02
+    off_r, d_r, i_r = built["orc"].ball_csr(q, r, True)
+    off, d, i = built["tree"].ball(torch.from_numpy(q), r, sorted=True)
+    assert_csr_equal(off_r, d_r, i_r, off.cpu().numpy(), d.cpu().numpy(), i.cpu().numpy(), sorted_got=True)
