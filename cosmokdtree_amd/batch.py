@@ -88,7 +88,7 @@ class BatchPlan:
 
     def __init__(self, forest: KDTree, nq_per_set, k_per_set, radius_per_set=None, sorted: bool = True,
                  threads: int = 256, blocks: int = 0, ball_capacity: int | None = None, knn_kernel: str | None = None,
-                 queries_per_wave: int = 64, stack_depth: int = 0, ball_kernel: str = "lane"):
+                 queries_per_wave: int = 64, stack_depth: int = 0, ball_kernel: str | None = None):
         self.forest = forest
         self.ball_kernel = ball_kernel
         self.knn_kernel, self.queries_per_wave, self.stack_depth = knn_kernel, int(queries_per_wave), int(stack_depth)
@@ -136,9 +136,10 @@ class BatchPlan:
 
     def _apply_config(self):
         kinds = {"lane": 0, "stack": 2}
-        if self.ball_kernel not in kinds:
-            raise ValueError("ball_kernel must be 'lane' or 'stack'")
-        check(lib.ckd_batch_config_ball(self._plan, kinds[self.ball_kernel]), "batch_config_ball")
+        if self.ball_kernel is not None:  # None: the plan's own default (by query density)
+            if self.ball_kernel not in kinds:
+                raise ValueError("ball_kernel must be 'lane' or 'stack'")
+            check(lib.ckd_batch_config_ball(self._plan, kinds[self.ball_kernel]), "batch_config_ball")
         if self.knn_kernel is None:  # the plan's own default (by query density)
             return
         if self.knn_kernel not in kinds:

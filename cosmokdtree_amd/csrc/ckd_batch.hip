@@ -92,10 +92,12 @@ int ckd_batch_plan(const ckd_tree *t, const int64_t *qoff, const int32_t *k, con
     b->nt = threads;
     b->has_ball = any_ball ? 1 : 0;
     b->has_knn = knn_off[ns] > 0 ? 1 : 0;
-    // default kNN kernel (measured on MI355X): shared walks pay when the
-    // queries are at least 1/20 of the points; sparser batches run one walk
-    // per query
+    // default kNN and ball kernels (measured on MI355X): shared walks pay
+    // when the queries are at least 1/20 of the points; sparser batches run
+    // one walk per query.  On 2e8 points in 4096 sets with queries 1/10 of
+    // the points, the shared ball walk took 190 ms against 461 ms.
     b->knn_kernel = qoff[ns] * 20 >= t->n ? CKD_KNN_STACK : CKD_KNN_LANE;
+    b->ball_kernel = b->knn_kernel;
     int dev_cu = 256;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, t->device) == hipSuccess) dev_cu = prop.multiProcessorCount;

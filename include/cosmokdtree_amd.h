@@ -111,11 +111,11 @@ int ckd_get_info(const ckd_tree *tree, ckd_tree_info *info);
  * The wave kernels count per wave: nodes visited and pops are wave totals
  * summed over the lanes.
  * A large batch is walked in the order of the leaves its queries fall in.
- * When nq equals the tree's point count, the library first checks on the
- * device whether query idx[r] is the tree's row r for every row (the
- * self-query of the tree's own points, in any array); if so the tree's row
- * order is that ordering and the locate pass is skipped.  Results do not
- * depend on which ordering ran.
+ * When nq equals the tree's point count (one point set), the library first
+ * checks on the device whether query idx[r] is the tree's row r, for every
+ * 16th row r (the self-query of the tree's own points, in any array); if
+ * so the tree's row order is that ordering and the locate pass is skipped.
+ * Results do not depend on which ordering ran.
  */
 int ckd_knn(const ckd_tree *tree, const void *queries, int64_t nq, int32_t k, int32_t sorted, void *dist,
             int64_t *idx, const float *seed_r, int64_t *stats, void *stream);
@@ -326,11 +326,12 @@ int ckd_batch_plan(const ckd_tree *forest, const int64_t *qoff, const int32_t *k
    CKD_KNN_LANE (one walk per query; the default for sparser batches).
    Results never depend on it. */
 int ckd_batch_config(ckd_batch *plan, int32_t knn_kernel, int32_t queries_per_wave, int32_t stack_depth);
-/* Ball kernel of a plan: CKD_KNN_LANE (one walk per query, the default) or
-   CKD_KNN_STACK (the 64 queries of a wave share one walk over the set's
-   wide records, as ckd_ball_count/fill do; used when the forest has wide
-   records and its depth fits the stack).  Results never depend on it; row
-   order within a query may differ. */
+/* Ball kernel of a plan: CKD_KNN_STACK (the 64 queries of a wave share one
+   walk over the set's wide records, as ckd_ball_count/fill do; used when
+   the forest has wide records and its depth fits the stack; the default
+   when the queries are at least 1/20 as many as the points) or
+   CKD_KNN_LANE (one walk per query; the default for sparser batches).
+   Results never depend on it; row order within a query may differ. */
 int ckd_batch_config_ball(ckd_batch *plan, int32_t ball_kernel);
 int ckd_batch_run(ckd_batch *plan, const void *queries, void *knn_dist, int64_t *knn_idx, int64_t *ball_off,
                   void *ball_dist, int64_t *ball_idx, void *stream);
