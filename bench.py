@@ -36,6 +36,7 @@ sys.path.insert(0, ROOT)
 METRIC = "kNN k=32 queries/s on 16.7M periodic pts, learned-seed vs plain; build Mpts/s"
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0  # MI355X dense bf16 MFMA
+PEAK_CLOCK_GHZ = 2.4  # MI355X peak engine clock
 
 
 def log(msg):
@@ -440,6 +441,19 @@ def main():
             tr = json.load(f)
         if tr.get("n") == N and tr.get("k") == K and tr.get("source_sha") == knn_source_sha():
             traffic = tr.get("seeded_bytes_per_launch" if use_learned else "plain_bytes_per_launch")
+            if not use_learned and tr.get("plain_valu_insts"):
+                # the kernel's other roofline: vector ALU issue.  A wave64
+                # FP32 instruction without packed math holds a SIMD for 4
+                # cycles (16 lanes a cycle); the counted instructions over
+                # the SIMDs, against the launch at the peak engine clock
+                simds = 4 * torch.cuda.get_device_properties(0).multi_processor_count
+                cyc = tr["plain_valu_insts"] * 4 / simds
+                extra["valu_issue"] = dict(valu_insts_per_launch=tr["plain_valu_insts"], simds=simds,
+                                           cycles_per_simd=cyc, kernel_ms=kernel_ms, clock_ghz=PEAK_CLOCK_GHZ,
+                                           frac=cyc / (kernel_ms * 1e-3 * PEAK_CLOCK_GHZ * 1e9),
+                                           source="profiles/knn_clustered_2e24_k32_counters.txt")
+                log(f"VALU issue of the traversal kernel: {cyc / 1e6:.1f} M cycles per SIMD in {kernel_ms:.2f} ms, "
+                    f"{100 * extra['valu_issue']['frac']:.0f} % of the SIMDs' cycles at {PEAK_CLOCK_GHZ} GHz")
         else:
             log("profiles/knn_traffic.json measured other kernel sources or sizes: traffic not reported")
 
