@@ -472,65 +472,72 @@ __global__ __launch_bounds__(QT) void k_ball_stage(TreeView<T> tv, int2 root, in
     if (active) counts[qi] = c;
 }
 
-// The staging records to their CSR places: one block per taken chunk, a
-// thread per batch record (its accepted rows in row order); the subtrees
-// taken whole are spread over the block afterwards.  The distances are
-// computed again in the walk's formula (same bits), their square roots
-// taken here.  Nothing is written when the staging area overflowed or the
-// total exceeds `cap` (the caller then fills the rows with the second walk).
+// The staging records to their CSR places.  The blocks stride over the
+// taken chunks; in a chunk, a thread per batch record (its accepted rows in
+// row order); the subtrees taken whole are spread over the block
+// afterwards.  The distances are computed again in the walk's formula (same
+// bits), their square roots taken here.  Nothing is written when the
+// staging area overflowed or the total exceeds `cap` (the caller then fills
+// the rows with the second walk).
+constexpr unsigned SCATTER_BLOCKS = 4096;
+
 template <typename T, int D, bool PER>
 __global__ __launch_bounds__(256) void k_ball_scatter(TreeView<T> tv, const T *queries, StageOut so, const int64_t *off,
                                                       int64_t nq, int64_t cap, T *dout, int64_t *iout) {
-    const int32_t c = blockIdx.x;
-    if ((uint32_t)c >= *so.next_chunk || c >= so.nchunks || *so.overflow || off[nq] > cap) return;
+    if (*so.overflow || off[nq] > cap) return;
+    const uint32_t taken = *so.next_chunk, used = taken < (uint32_t)so.nchunks ? taken : (uint32_t)so.nchunks;
     __shared__ int4 ranges[STAGE_CHUNK];
     __shared__ int nr;
-    if (threadIdx.x == 0) nr = 0;
-    __syncthreads();
-    const int fill = so.chunk_fill[c];
-    CKD_DASSERT(fill >= 0 && fill <= STAGE_CHUNK);
 #pragma unroll 1
-    for (int r = threadIdx.x; r < fill; r += blockDim.x) {
-        const int4 e = so.rec[c * STAGE_CHUNK + r];
-        if ((uint32_t)e.x & STAGE_RANGE) {
-            ranges[atomicAdd(&nr, 1)] = e;
-            continue;
-        }
-        CKD_DASSERT(e.x >= 0 && e.x < nq && e.z >= 0 && e.z < tv.nrows);
-        int64_t o = off[e.x] + e.y;
-        T q[D];
+    for (uint32_t c = blockIdx.x; c < used; c += gridDim.x) {
+        if (threadIdx.x == 0) nr = 0;
+        __syncthreads();
+        const int fill = so.chunk_fill[c];
+        CKD_DASSERT(fill >= 0 && fill <= STAGE_CHUNK);
+#pragma unroll 1
+        for (int r = threadIdx.x; r < fill; r += blockDim.x) {
+            const int4 e = so.rec[(int64_t)c * STAGE_CHUNK + r];
+            if ((uint32_t)e.x & STAGE_RANGE) {
+                ranges[atomicAdd(&nr, 1)] = e;
+                continue;
+            }
+            CKD_DASSERT(e.x >= 0 && e.x < nq && e.z >= 0 && e.z < tv.nrows);
+            int64_t o = off[e.x] + e.y;
+            T q[D];
 #pragma unroll
-        for (int d = 0; d < D; d++) q[d] = queries[(int64_t)e.x * D + d];
-        uint32_t mk = (uint32_t)e.w;
-        while (mk) {
-            const int j = __builtin_ctz(mk);
-            mk &= mk - 1;
-            CKD_DASSERT(o < off[e.x + 1]);
-            T x[D];
-            load_pt<T, D>(tv.pts, e.z + j, x);
-            dout[o] = sqrt_rn<T>(dist2<T, D, PER>(x, q, tv.L));
-            iout[o] = load_idx(tv.idx, tv.idx64, e.z + j);
-            o++;
+            for (int d = 0; d < D; d++) q[d] = queries[(int64_t)e.x * D + d];
+            uint32_t mk = (uint32_t)e.w;
+            while (mk) {
+                const int j = __builtin_ctz(mk);
+                mk &= mk - 1;
+                CKD_DASSERT(o < off[e.x + 1]);
+                T x[D];
+                load_pt<T, D>(tv.pts, e.z + j, x);
+                dout[o] = sqrt_rn<T>(dist2<T, D, PER>(x, q, tv.L));
+                iout[o] = load_idx(tv.idx, tv.idx64, e.z + j);
+                o++;
+            }
         }
-    }
-    __syncthreads();
+        __syncthreads();
 #pragma unroll 1
-    for (int k = 0; k < nr; k++) {
-        const int4 e = ranges[k];
-        const int32_t qi = (int32_t)((uint32_t)e.x & ~STAGE_RANGE);
-        CKD_DASSERT(qi >= 0 && qi < nq && e.z >= 0 && e.z + e.w <= tv.nrows);
-        const int64_t o = off[qi] + e.y;
-        CKD_DASSERT(o >= off[qi] && o + e.w <= off[qi + 1]);
-        T q[D];
+        for (int k = 0; k < nr; k++) {
+            const int4 e = ranges[k];
+            const int32_t qi = (int32_t)((uint32_t)e.x & ~STAGE_RANGE);
+            CKD_DASSERT(qi >= 0 && qi < nq && e.z >= 0 && e.z + e.w <= tv.nrows);
+            const int64_t o = off[qi] + e.y;
+            CKD_DASSERT(o >= off[qi] && o + e.w <= off[qi + 1]);
+            T q[D];
 #pragma unroll
-        for (int d = 0; d < D; d++) q[d] = queries[(int64_t)qi * D + d];
+            for (int d = 0; d < D; d++) q[d] = queries[(int64_t)qi * D + d];
 #pragma unroll 1
-        for (int j = threadIdx.x; j < e.w; j += blockDim.x) {
-            T x[D];
-            load_pt<T, D>(tv.pts, e.z + j, x);
-            dout[o + j] = sqrt_rn<T>(dist2<T, D, PER>(x, q, tv.L));
-            iout[o + j] = load_idx(tv.idx, tv.idx64, e.z + j);
+            for (int j = threadIdx.x; j < e.w; j += blockDim.x) {
+                T x[D];
+                load_pt<T, D>(tv.pts, e.z + j, x);
+                dout[o + j] = sqrt_rn<T>(dist2<T, D, PER>(x, q, tv.L));
+                iout[o + j] = load_idx(tv.idx, tv.idx64, e.z + j);
+            }
         }
+        __syncthreads();  // nr and ranges are reused by the next chunk
     }
 }
 
@@ -1007,8 +1014,9 @@ int ball_search_launch(const ckd_tree *t, const T *q, int64_t nq, T r2, int64_t 
         }
         CKD_HIP(hipGetLastError());
         if ((rc = scan_i64(counts, off, nq, true, scr, s))) return rc;
-        if (t->periodic) k_ball_scatter<T, D, true><<<(unsigned)nchunks, 256, 0, s>>>(tv, q, so, off, nq, cap, dist, idx);
-        else k_ball_scatter<T, D, false><<<(unsigned)nchunks, 256, 0, s>>>(tv, q, so, off, nq, cap, dist, idx);
+        const unsigned gs = nchunks < SCATTER_BLOCKS ? (unsigned)nchunks : SCATTER_BLOCKS;
+        if (t->periodic) k_ball_scatter<T, D, true><<<gs, 256, 0, s>>>(tv, q, so, off, nq, cap, dist, idx);
+        else k_ball_scatter<T, D, false><<<gs, 256, 0, s>>>(tv, q, so, off, nq, cap, dist, idx);
         CKD_HIP(hipGetLastError());
         CKD_HIP(hipMemcpyAsync(total, off + nq, sizeof(int64_t), hipMemcpyDeviceToHost, s));
         CKD_HIP(hipMemcpyAsync(overflow, so.overflow, sizeof(int32_t), hipMemcpyDeviceToHost, s));
