@@ -591,13 +591,33 @@ __global__ void k_leaf_boxes(const NodeRec<T> *nodes, const Range *range, int64_
         mn[d] = T(INFINITY);
         mx[d] = T(-INFINITY);
     }
-    for (int64_t p = r.start; p < r.start + r.count; p++)
+    // four rows' loads in flight together; past the leaf's last row the
+    // last row is read again, which leaves the extremes as they are
+    const int64_t e = r.start + r.count;
+    for (int64_t p = r.start; p < e; p += 4) {
+        T x[4][D];
 #pragma unroll
-        for (int d = 0; d < D; d++) {
-            const T x = F[p * PS + d];
-            mn[d] = x < mn[d] ? x : mn[d];
-            mx[d] = x > mx[d] ? x : mx[d];
+        for (int u = 0; u < 4; u++) {
+            const int64_t q = p + u < e ? p + u : e - 1;
+            if constexpr (sizeof(T) == 4 && PS == 4) {  // one 16-byte load per row (rows are 16-byte aligned)
+                const float4 v = *(const float4 *)(F + q * PS);
+                x[u][0] = v.x;
+                if constexpr (D > 1) x[u][1] = v.y;
+                if constexpr (D > 2) x[u][2] = v.z;
+                if constexpr (D > 3) x[u][3] = v.w;
+            } else {
+#pragma unroll
+                for (int d = 0; d < D; d++) x[u][d] = F[q * PS + d];
+            }
         }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+#pragma unroll
+            for (int d = 0; d < D; d++) {
+                mn[d] = x[u][d] < mn[d] ? x[u][d] : mn[d];
+                mx[d] = x[u][d] > mx[d] ? x[u][d] : mx[d];
+            }
+    }
 #pragma unroll
     for (int d = 0; d < D; d++) {
         dbox[v * 2 * D + d] = mn[d];
@@ -703,12 +723,24 @@ __global__ void k_fill_wide(const NodeRec<T> *nodes, const T *dbox, const Range 
     const Range rv = range[v];
     wrange[widx[v]] = make_int2((int32_t)rv.start, (int32_t)rv.count);
     char *base = wide + widx[v] * wide_bytes<T, D>();
-    T *bx = (T *)base;
-#pragma unroll
-    for (int c = 0; c < 2; c++)
-#pragma unroll
-        for (int d = 0; d < 2 * D; d++) bx[c * 2 * D + d] = dbox[(int64_t)(nr.child + c) * 2 * D + d];
     const int2 p0 = child_pair(nodes, range, widx, nr.child), p1 = child_pair(nodes, range, widx, nr.child + 1);
+    if constexpr (sizeof(T) == 4 && D == 3) {
+        // the two children's boxes are 12 consecutive floats (8-byte aligned);
+        // the 64-byte record takes them as three 16-byte stores
+        const float2 *src = (const float2 *)(dbox + (int64_t)nr.child * 2 * D);
+        float2 h[6];
+#pragma unroll
+        for (int i = 0; i < 6; i++) h[i] = src[i];
+        float4 *dst = (float4 *)base;
+#pragma unroll
+        for (int i = 0; i < 3; i++) dst[i] = make_float4(h[2 * i].x, h[2 * i].y, h[2 * i + 1].x, h[2 * i + 1].y);
+    } else {
+        T *bx = (T *)base;
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int d = 0; d < 2 * D; d++) bx[c * 2 * D + d] = dbox[(int64_t)(nr.child + c) * 2 * D + d];
+    }
     *(int4 *)(base + 4 * D * sizeof(T)) = make_int4(p0.x, p0.y, p1.x, p1.y);
 }
 
