@@ -72,3 +72,21 @@ def test_ball_self_order(built, variant):
     off_r, d_r, i_r = built["orc"].ball_csr(q, r, True)
     off, d, i = built["tree"].ball(torch.from_numpy(q), r, sorted=True)
     assert_csr_equal(off_r, d_r, i_r, off.cpu().numpy(), d.cpu().numpy(), i.cpu().numpy(), sorted_got=True)
+
+
+def test_self_order_idx64(built, ckd):
+    """The self-query ordering reads the stored original indices: with
+    64-bit indices (LONGINT trees) the rows equal the 32-bit tree's and the
+    oracle's."""
+    pts, L = built["pts"], built["L"]
+    t64 = ckd.kdtree._build(pts, 16, L, idx_bits=64)
+    try:
+        assert t64.info().idx_bits == 64
+        P = torch.from_numpy(pts).cuda()
+        d64, i64 = t64.knn(P, 32, sorted=True)
+        d32, i32 = built["tree"].knn(P, 32, sorted=True)
+        assert torch.equal(d32, d64) and torch.equal(i32, i64)
+        d_ref, i_ref = built["orc"].knn(pts[::97], 32, True)
+        assert_knn_rows(pts, pts[::97], 32, d_ref, i_ref, d64.cpu().numpy()[::97], i64.cpu().numpy()[::97], L=L)
+    finally:
+        t64.free()
