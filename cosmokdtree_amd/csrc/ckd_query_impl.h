@@ -244,7 +244,10 @@ __global__ __launch_bounds__(QT, 3) void k_knn_w(TreeView<T> tv, const T *querie
 // At k <= 32 in float32 with one lane per query (91 registers) the LDS sets
 // the occupancy: 12 slots per lane (24 KB) and the 4 KB stack fit 5 blocks
 // per CU, 5 waves per SIMD; 16 slots fitted 4, and the self-query of the
-// clustered 2^24 box took 32.6 ms instead of 28.5.
+// clustered 2^24 box took 32.6 ms instead of 28.5.  In double precision at
+// k <= 32 the launch bound asks for 4 waves per SIMD (128 registers, no
+// scratch): the same self-query in float64 took 47.3 ms instead of 54.4 at
+// 3 waves (143 registers); 5 waves spilled (720 ms).
 // G lanes per query (1, 2, 4, ..., 64; 64 / G queries per wave): the grid
 // covers nq * G lanes.  GC = 1 compiles the one-lane-per-query walk with G
 // folded away (the group scans cost registers and time even when G is 1 at
@@ -252,7 +255,7 @@ __global__ __launch_bounds__(QT, 3) void k_knn_w(TreeView<T> tv, const T *querie
 // work counters away (calls without `stats`): live, their 64-bit sums held
 // ten registers and were spilled to scratch inside the walk.
 template <typename T, int D, bool PER, int KMAX, int SD, int GC, bool ST>
-__global__ __launch_bounds__(QT, sizeof(T) == 4 ? (KMAX <= 32 && GC == 1 ? 5 : 4) : 3) void k_knn_s(TreeView<T> tv, int2 root, const T *queries, int64_t nq,
+__global__ __launch_bounds__(QT, KMAX <= 32 && GC == 1 ? (sizeof(T) == 4 ? 5 : 4) : (sizeof(T) == 4 ? 4 : 3)) void k_knn_s(TreeView<T> tv, int2 root, const T *queries, int64_t nq,
                                               const int64_t *nq_dev, int k, int sorted, const float *seed,
                                               const int32_t *order, const int32_t *leaf_of, T *dout,
                                               int64_t *iout, int64_t *stats, Retry rt, int Grt) {
