@@ -35,6 +35,8 @@ constexpr int CPB = BT / WAVE;    // chunks (or nodes) per workgroup, one per wa
 constexpr int64_t NODE_THREAD_MIN_CH = 4;  // decide with one thread per node when chunks <= 4 x nodes
 constexpr int RU = 4;             // rounds of 64 points whose loads a chunk wave issues together
 constexpr int STATS_G = 16;       // lanes per chunk of the extremes pass on levels of small nodes
+constexpr int64_t DECIDE_BLOCK_MAX = 16;  // levels of at most this many nodes decide with a workgroup per node
+constexpr int64_t DECIDE_THREAD_MIN = 8192;  // a thread per node from this many nodes (fewer leave most CUs idle)
 
 struct Chunk {
     int64_t begin, end;  // positions in the working buffers
@@ -341,6 +343,40 @@ __global__ __launch_bounds__(BT) void k_node_decide(const int32_t *act, int64_t 
     }
     wave_reduce_ext<T, D>(e);
     if (lane == 0) decide_node<T, D>(a, act[a], e, nodes, bbox, dbox, dec, root_from_data);
+}
+
+// One workgroup per active node (the top levels: few nodes with thousands
+// of chunks each, where one wave per node waited on a chain of 32 rounds of
+// loads at the root: 67 -> 27 us there; from 32 nodes on, a wave per node
+// is faster).  The merge keeps the smallest (value, position), so its order
+// does not change the result.
+template <typename T, int D>
+__global__ __launch_bounds__(BT) void k_node_decide_b(const int32_t *act, int64_t nact, const int64_t *chunk_base,
+                                                      const int64_t *nch, const Ext<T> *cstat, NodeRec<T> *nodes,
+                                                      T *bbox, T *dbox, Decision<T> *dec, int root_from_data) {
+    __shared__ Ext<T> lds[BT / WAVE][D];
+    const int64_t a = blockIdx.x;
+    Ext<T> e[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) e[d] = ext_empty<T>();
+    const int64_t c0 = chunk_base[a], c1 = c0 + nch[a];
+    for (int64_t base = c0; base < c1; base += RU * BT) {
+        Ext<T> x[RU][D];  // RU rounds' loads in flight together
+#pragma unroll
+        for (int u = 0; u < RU; u++) {
+            const int64_t c = base + u * BT + threadIdx.x;
+            const int64_t q = c < c1 ? c : base;
+#pragma unroll
+            for (int d = 0; d < D; d++) x[u][d] = cstat[q * D + d];
+        }
+#pragma unroll
+        for (int u = 0; u < RU; u++)
+            if (base + u * BT + threadIdx.x < c1)
+#pragma unroll
+                for (int d = 0; d < D; d++) ext_merge(e[d], x[u][d]);
+    }
+    block_reduce_ext<T, D>(e, lds);
+    if (threadIdx.x == 0) decide_node<T, D>(a, act[a], e, nodes, bbox, dbox, dec, root_from_data);
 }
 
 // One thread per active node, merging its chunks in order (levels of many
@@ -1035,8 +1071,12 @@ int build_impl(ckd_tree *t, const T *in, hipStream_t s) {
                     chunks.as<Chunk>(), d_ntot, Pc, n, cstat.as<Ext<T>>());
             else
                 k_chunk_stats<T, D><<<gch, BT, 0, s>>>(chunks.as<Chunk>(), d_ntot, Pc, n, cstat.as<Ext<T>>());
-            if (small_nodes)  // one thread per node
+            if (small_nodes && nact >= DECIDE_THREAD_MIN)  // one thread per node
                 k_node_decide_t<T, D><<<(unsigned)ceil_div(nact, BT), BT, 0, s>>>(
+                    actc, nact, d_cb, d_nch, cstat.as<Ext<T>>(), (NodeRec<T> *)t->nodes, (T *)t->bbox, (T *)t->dbox,
+                    dec.as<Decision<T>>(), level == 0 && !t->periodic);
+            else if (nact <= DECIDE_BLOCK_MAX)  // a workgroup per node
+                k_node_decide_b<T, D><<<(unsigned)nact, BT, 0, s>>>(
                     actc, nact, d_cb, d_nch, cstat.as<Ext<T>>(), (NodeRec<T> *)t->nodes, (T *)t->bbox, (T *)t->dbox,
                     dec.as<Decision<T>>(), level == 0 && !t->periodic);
             else
