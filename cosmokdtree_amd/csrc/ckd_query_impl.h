@@ -751,12 +751,16 @@ inline int learned_buffers(Scratch &sc, int64_t nq, uint16_t **X, float **seeds)
 // seeds, 3-D trees) the same pass writes each query's features.  A batch as
 // large as the tree's one point set first tries the tree's own row order
 // (k_self_order); the locate pass and the counting sort then do their work
-// only if the queries are not the tree's points, as a device count decides,
-// so the host does not wait.
+// only if the queries are not the tree's points, as a device flag decides,
+// so the host does not wait.  The leaf of a row need not be the leaf the
+// descent by split values reaches (a point on a split plane may have been
+// placed on either side): only callers that use leaf_of as a hint (the
+// stack walks' prefill, the features) may take it (self_ok); the walk of
+// k_knn climbs from its leaf and needs the descent's leaf.
 template <typename T, int D>
 int order_queries(const ckd_tree *t, const TreeView<T> &tv, const T *q, int64_t nq, const int32_t **order,
                   const int32_t **leaf_of, Scratch &sc, hipStream_t s, const Learned *ln = nullptr,
-                  uint16_t *X = nullptr) {
+                  uint16_t *X = nullptr, bool self_ok = true) {
     const unsigned g = (unsigned)ceil_div(nq, QT);
     const size_t a = 3 * sizeof(int32_t) * (size_t)nq;
     void *ws = nullptr;
@@ -768,7 +772,7 @@ int order_queries(const ckd_tree *t, const TreeView<T> &tv, const T *q, int64_t 
     int64_t *off = cnt + t->nnodes + 1;
     void *scr = off + t->nnodes + 1;
     const int32_t *self = nullptr;
-    if (nq == t->n && t->nsets == 1) {
+    if (self_ok && nq == t->n && t->nsets == 1) {
         k_fill_i32<<<1, 64, 0, s>>>(differ, 1, 0);
         k_row_leaf<T><<<(unsigned)ceil_div(t->nnodes, QT), QT, 0, s>>>(tv, rk);
         k_self_order<T, D><<<g, QT, 0, s>>>(tv, q, rk, lo, ord, differ);
@@ -832,7 +836,10 @@ int knn_launch(const ckd_tree *t, const T *q, int64_t nq, int k, int sorted, con
                     feats_done = true;
                 }
             }
-            if ((rc = order_queries<T, D>(t, tv, q, nq, &order, &leaf_of, sc, s, ln, feats_done ? X : nullptr)))
+            // the self-query ordering's leaves serve the stack walk's prefill;
+            // the per-query walk (k_knn) needs the descent's leaves
+            if ((rc = order_queries<T, D>(t, tv, q, nq, &order, &leaf_of, sc, s, ln, feats_done ? X : nullptr,
+                                          kind == CKD_KNN_STACK)))
                 return rc;
         }
         if (ln) {

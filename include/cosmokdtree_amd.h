@@ -114,7 +114,8 @@ int ckd_get_info(const ckd_tree *tree, ckd_tree_info *info);
  * When nq equals the tree's point count (one point set), the library first
  * checks on the device whether query idx[r] is the tree's row r, for every
  * 16th row r (the self-query of the tree's own points, in any array); if
- * so the tree's row order is that ordering and the locate pass is skipped.
+ * so the tree's row order is that ordering and the locate pass is skipped
+ * (for the walk shared per wave; the walk per query keeps the locate pass).
  * Results do not depend on which ordering ran.
  */
 int ckd_knn(const ckd_tree *tree, const void *queries, int64_t nq, int32_t k, int32_t sorted, void *dist,

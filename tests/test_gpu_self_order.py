@@ -90,3 +90,29 @@ def test_self_order_idx64(built, ckd):
         assert_knn_rows(pts, pts[::97], 32, d_ref, i_ref, d64.cpu().numpy()[::97], i64.cpu().numpy()[::97], L=L)
     finally:
         t64.free()
+
+
+@pytest.mark.parametrize("ndim,kernel", [(3, "lane"), (4, "auto"), (3, "auto")])
+def test_self_order_points_on_split_planes(ckd, ndim, kernel):
+    """Points on a 3^ndim lattice, dozens to hundreds at each site: the
+    sliding midpoint peels one point of a site at a time into a leaf of its
+    own, left of a split equal to its coordinate, while the descent by split
+    values sends that point right.  The leaf holding a query's row is then
+    not the leaf the per-query walk (which climbs from its leaf) must start
+    from, so that walk takes the descent's leaves; the stack walk uses the
+    row's leaf only to prefill.  Self-queries through the per-query kernel
+    (forced, or the only one for ndim > 3) and through the default kernel
+    must give the oracle's rows."""
+    rng = np.random.default_rng(17)
+    pts = rng.integers(0, 3, (5000, ndim)).astype(np.float64)
+    tree = ckd.kdtree._build(pts, 16, None)
+    orc = OracleTree(pts, 16, None)
+    try:
+        if kernel != "auto":
+            tree.knn_config(kernel)
+        d, i = tree.knn(torch.from_numpy(pts), 16, sorted=True)
+        d_ref, i_ref = orc.knn(pts, 16, True)
+        assert_knn_rows(pts, pts, 16, d_ref, i_ref, d.cpu().numpy(), i.cpu().numpy())
+    finally:
+        tree.free()
+        orc.close()
