@@ -116,3 +116,27 @@ def test_self_order_points_on_split_planes(ckd, ndim, kernel):
     finally:
         tree.free()
         orc.close()
+
+
+@pytest.mark.parametrize("ndim", [1, 2, 3])
+@pytest.mark.parametrize("periodic", [False, True])
+@pytest.mark.parametrize("kernel,qpw", [("auto", 0), ("stack", 0), ("stack", 4), ("wave", 0), ("lane", 0)])
+def test_self_query_every_kernel(ckd, ndim, periodic, kernel, qpw):
+    """Self-queries (the tree's own points, a fresh array) through every kNN
+    kernel and periodicity, float32, with k inside one list and past it;
+    coarse coordinates put many points on split planes."""
+    rng = np.random.default_rng(100 + ndim)
+    n = 6000
+    L = np.full(ndim, 16.0) if periodic else None
+    pts = (np.floor(rng.random((n, ndim)) * 64) / 4 - (8.0 if periodic else 0.0)).astype(np.float32)
+    tree = ckd.kdtree._build(pts, 16, L)
+    orc = OracleTree(pts, 16, L)
+    try:
+        tree.knn_config(kernel, 0, qpw)
+        for k in (8, 40):
+            d, i = tree.knn(torch.from_numpy(pts.copy()), k, sorted=True)
+            d_ref, i_ref = orc.knn(pts, k, True)
+            assert_knn_rows(pts, pts, k, d_ref, i_ref, d.cpu().numpy(), i.cpu().numpy(), L=L)
+    finally:
+        tree.free()
+        orc.close()
