@@ -315,17 +315,37 @@ __device__ __forceinline__ void decide_node(int64_t a, int32_t id, const Ext<T> 
 
 // One wave per active node: its lanes merge the node's chunk statistics,
 // lane 0 applies the split rule (levels of few nodes with many chunks).
+// With slot_of, node a is the child slot slot_of[a] = 2 x parent + side of
+// the previous level, and its statistics are that level's chunk records of
+// the parent (chunk_base, nch of that level; cstat left, cstat_r right).
+template <typename T, int D>
+__device__ __forceinline__ void decide_chunks(int64_t a, const int64_t *chunk_base, const int64_t *nch,
+                                              const Ext<T> *cstat, const int64_t *slot_of, const Ext<T> *cstat_r,
+                                              int64_t &c0, int64_t &c1, const Ext<T> *&cs) {
+    int64_t pa = a;
+    cs = cstat;
+    if (slot_of) {
+        const int64_t m = slot_of[a];
+        pa = m >> 1;
+        cs = (m & 1) ? cstat_r : cstat;
+    }
+    c0 = chunk_base[pa];
+    c1 = c0 + nch[pa];
+}
+
 template <typename T, int D>
 __global__ __launch_bounds__(BT) void k_node_decide(const int32_t *act, int64_t nact, const int64_t *chunk_base,
                                                     const int64_t *nch, const Ext<T> *cstat, NodeRec<T> *nodes,
-                                                    T *bbox, T *dbox, Decision<T> *dec, int root_from_data) {
+                                                    T *bbox, T *dbox, Decision<T> *dec, int root_from_data,
+                                                    const int64_t *slot_of = nullptr, const Ext<T> *cstat_r = nullptr) {
     const int64_t a = (int64_t)blockIdx.x * CPB + threadIdx.x / WAVE;
     if (a >= nact) return;  // whole wave
     const int lane = threadIdx.x & (WAVE - 1);
     Ext<T> e[D];
 #pragma unroll
     for (int d = 0; d < D; d++) e[d] = ext_empty<T>();
-    const int64_t c0 = chunk_base[a], c1 = c0 + nch[a];
+    int64_t c0, c1;
+    decide_chunks<T, D>(a, chunk_base, nch, cstat, slot_of, cstat_r, c0, c1, cstat);
     for (int64_t base = c0; base < c1; base += RU * WAVE) {
         Ext<T> x[RU][D];  // RU rounds' loads in flight together
 #pragma unroll
@@ -353,13 +373,16 @@ __global__ __launch_bounds__(BT) void k_node_decide(const int32_t *act, int64_t 
 template <typename T, int D>
 __global__ __launch_bounds__(BT) void k_node_decide_b(const int32_t *act, int64_t nact, const int64_t *chunk_base,
                                                       const int64_t *nch, const Ext<T> *cstat, NodeRec<T> *nodes,
-                                                      T *bbox, T *dbox, Decision<T> *dec, int root_from_data) {
+                                                      T *bbox, T *dbox, Decision<T> *dec, int root_from_data,
+                                                      const int64_t *slot_of = nullptr,
+                                                      const Ext<T> *cstat_r = nullptr) {
     __shared__ Ext<T> lds[BT / WAVE][D];
     const int64_t a = blockIdx.x;
     Ext<T> e[D];
 #pragma unroll
     for (int d = 0; d < D; d++) e[d] = ext_empty<T>();
-    const int64_t c0 = chunk_base[a], c1 = c0 + nch[a];
+    int64_t c0, c1;
+    decide_chunks<T, D>(a, chunk_base, nch, cstat, slot_of, cstat_r, c0, c1, cstat);
     for (int64_t base = c0; base < c1; base += RU * BT) {
         Ext<T> x[RU][D];  // RU rounds' loads in flight together
 #pragma unroll
@@ -435,12 +458,13 @@ __global__ __launch_bounds__(BT) void k_chunk_countleft(const Chunk *chunks, con
 // ballot rank, added to the left (right) points of earlier rounds, earlier
 // chunks of the node (cleft_off) and, for the right side, the node's left
 // count.  Reads and writes of a round are contiguous runs.
-template <typename T, int D, typename IdxT>
+template <typename T, int D, typename IdxT, bool FUSE = false>
 __global__ __launch_bounds__(BT) void k_chunk_scatter(const Chunk *chunks, const int64_t *nchunks,
                                                       const int64_t *chunk_base, const int64_t *nch,
                                                       const int32_t *act, const Range *range, const T *P,
                                                       const IdxT *I, T *P2, IdxT *I2, int64_t n,
-                                                      const Decision<T> *dec, const int64_t *cleft_off) {
+                                                      const Decision<T> *dec, const int64_t *cleft_off,
+                                                      Ext<T> *cs_l = nullptr, Ext<T> *cs_r = nullptr) {
     const int64_t ci = (int64_t)blockIdx.x * CPB + threadIdx.x / WAVE;
     if (ci >= *nchunks) return;  // whole wave
     const int lane = threadIdx.x & (WAVE - 1);
@@ -453,6 +477,11 @@ __global__ __launch_bounds__(BT) void k_chunk_scatter(const Chunk *chunks, const
     const int64_t n_left = cleft_off[first + nch[ch.act]] - cleft_off[first];
     const int64_t right_base = nstart + n_left + (ch.begin - nstart) - (cleft_off[ci] - cleft_off[first]);
     int64_t lacc = 0, racc = 0;
+    Ext<T> el[D], er[D];  // FUSE: this lane's extremes of each child, positions in P2
+    if constexpr (FUSE) {
+#pragma unroll
+        for (int d = 0; d < D; d++) el[d] = er[d] = ext_empty<T>();
+    }
     for (int64_t base = ch.begin; base < ch.end; base += RU * WAVE) {
         T x[RU][D];  // RU rounds' loads in flight together
         IdxT iv[RU];
@@ -482,10 +511,29 @@ __global__ __launch_bounds__(BT) void k_chunk_scatter(const Chunk *chunks, const
 #pragma unroll
                 for (int d = 0; d < D; d++) P2[(int64_t)d * n + dst] = x[u][d];
                 I2[dst] = iv[u];
+                if constexpr (FUSE) {
+#pragma unroll
+                    for (int d = 0; d < D; d++) {
+                        Ext<T> pt;
+                        pt.mn = pt.mx = x[u][d];
+                        pt.mnpos = pt.mxpos = dst;
+                        ext_merge(f ? el[d] : er[d], pt);
+                    }
+                }
             }
             lacc += nl;
             racc += cnt - nl;
         }
+    }
+    if constexpr (FUSE) {  // the chunk's record of each child's extremes
+        wave_reduce_ext<T, D>(el);
+        wave_reduce_ext<T, D>(er);
+        if (lane == 0)
+#pragma unroll
+            for (int d = 0; d < D; d++) {
+                cs_l[ci * D + d] = el[d];
+                cs_r[ci * D + d] = er[d];
+            }
     }
 }
 
@@ -525,10 +573,13 @@ __global__ void k_make_children(const int32_t *act, int64_t nact, const Decision
 }
 
 __global__ void k_compact_active(const int64_t *child_active, const int64_t *pos, int64_t m, int32_t next_id,
-                                 int32_t *next_act) {
+                                 int32_t *next_act, int64_t *slot_of = nullptr) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m) return;
-    if (child_active[i]) next_act[pos[i]] = next_id + (int32_t)i;
+    if (child_active[i]) {
+        next_act[pos[i]] = next_id + (int32_t)i;
+        if (slot_of) slot_of[pos[i]] = i;  // the child slot (2 x parent + side) of each next active node
+    }
 }
 
 // Copy the points of the leaves created this level (children that are not
@@ -1016,6 +1067,18 @@ int build_impl(ckd_tree *t, const T *in, hipStream_t s) {
             (rc = child_pos.alloc(sizeof(int64_t) * (2 * nact_max + 1))) ||
             (rc = scratch.alloc(scan_tmp_bytes(2 * nact_max + nchunk_max + 2))))
             return rc;
+        // the children's extremes from the partition of the levels of large
+        // nodes (fewer than n / (3 CH) + 2 nodes: 4 nodes < n / CH + nodes +
+        // 1 chunk slots): chunk records of the right children beside cstat
+        // (left), the level's chunk bases kept for the next level's
+        // decisions, and each next active node's child slot
+        const int64_t fuse_nodes = n / (3 * CH) + 2;
+        DevBuf cstat_r(s), chunk_base2(s), nch2(s), slot_of(s);
+        if ((rc = cstat_r.alloc(sizeof(Ext<T>) * D * (n / CH + fuse_nodes + 1))) ||
+            (rc = chunk_base2.alloc(sizeof(int64_t) * (nact_max + 1))) || (rc = nch2.alloc(sizeof(int64_t) * nact_max)) ||
+            (rc = slot_of.alloc(sizeof(int64_t) * 2 * fuse_nodes)))
+            return rc;
+        bool fused_in = false;  // this level's extremes are the previous level's chunk records
         T *Pc = P.as<T>(), *Pn = P2.as<T>();
         IdxT *Ic = I.as<IdxT>(), *In = I2.as<IdxT>();
         int32_t *actc = act.as<int32_t>(), *actn = act2.as<int32_t>();
@@ -1055,7 +1118,12 @@ int build_impl(ckd_tree *t, const T *in, hipStream_t s) {
             }
             const unsigned ga = (unsigned)ceil_div(nact, 256);
             const int64_t nub = n / CH + nact + 1;  // upper bound of chunks this level
-            int64_t *d_nch = nch.as<int64_t>(), *d_cb = chunk_base.as<int64_t>();
+            // chunk bases alternate between two buffers: a fused level's
+            // decisions read the previous level's
+            int64_t *d_nch = (level & 1) ? nch2.as<int64_t>() : nch.as<int64_t>();
+            int64_t *d_cb = (level & 1) ? chunk_base2.as<int64_t>() : chunk_base.as<int64_t>();
+            const int64_t *p_nch = (level & 1) ? nch.as<int64_t>() : nch2.as<int64_t>();
+            const int64_t *p_cb = (level & 1) ? chunk_base.as<int64_t>() : chunk_base2.as<int64_t>();
             k_chunk_count<<<ga, 256, 0, s>>>(actc, nact, t->range, d_nch);
             if ((rc = scan_i64(d_nch, d_cb, nact, true, scratch.p, s))) break;
             // levels of small nodes: at most a few chunks per node
@@ -1066,36 +1134,58 @@ int build_impl(ckd_tree *t, const T *in, hipStream_t s) {
                 k_chunk_fill<<<(unsigned)nact, 64, 0, s>>>(actc, t->range, d_cb, chunks.as<Chunk>());
             const int64_t *d_ntot = d_cb + nact;  // total chunks (device)
             const unsigned gch = (unsigned)ceil_div(nub, CPB);  // one wave per chunk slot
-            if (small_nodes)  // 16 lanes per chunk
-                k_chunk_stats_g<T, D, STATS_G><<<(unsigned)ceil_div(nub * STATS_G, BT), BT, 0, s>>>(
-                    chunks.as<Chunk>(), d_ntot, Pc, n, cstat.as<Ext<T>>());
-            else
-                k_chunk_stats<T, D><<<gch, BT, 0, s>>>(chunks.as<Chunk>(), d_ntot, Pc, n, cstat.as<Ext<T>>());
-            if (small_nodes && nact >= DECIDE_THREAD_MIN)  // one thread per node
-                k_node_decide_t<T, D><<<(unsigned)ceil_div(nact, BT), BT, 0, s>>>(
-                    actc, nact, d_cb, d_nch, cstat.as<Ext<T>>(), (NodeRec<T> *)t->nodes, (T *)t->bbox, (T *)t->dbox,
-                    dec.as<Decision<T>>(), level == 0 && !t->periodic);
-            else if (nact <= DECIDE_BLOCK_MAX)  // a workgroup per node
-                k_node_decide_b<T, D><<<(unsigned)nact, BT, 0, s>>>(
-                    actc, nact, d_cb, d_nch, cstat.as<Ext<T>>(), (NodeRec<T> *)t->nodes, (T *)t->bbox, (T *)t->dbox,
-                    dec.as<Decision<T>>(), level == 0 && !t->periodic);
-            else
-                k_node_decide<T, D><<<(unsigned)ceil_div(nact, CPB), BT, 0, s>>>(
-                    actc, nact, d_cb, d_nch, cstat.as<Ext<T>>(), (NodeRec<T> *)t->nodes, (T *)t->bbox, (T *)t->dbox,
-                    dec.as<Decision<T>>(), level == 0 && !t->periodic);
+            if (fused_in) {  // the previous level's chunk records of each node's parent
+                const int64_t *so = slot_of.as<int64_t>();
+                if (nact <= DECIDE_BLOCK_MAX)
+                    k_node_decide_b<T, D><<<(unsigned)nact, BT, 0, s>>>(
+                        actc, nact, p_cb, p_nch, cstat.as<Ext<T>>(), (NodeRec<T> *)t->nodes, (T *)t->bbox,
+                        (T *)t->dbox, dec.as<Decision<T>>(), level == 0 && !t->periodic, so, cstat_r.as<Ext<T>>());
+                else
+                    k_node_decide<T, D><<<(unsigned)ceil_div(nact, CPB), BT, 0, s>>>(
+                        actc, nact, p_cb, p_nch, cstat.as<Ext<T>>(), (NodeRec<T> *)t->nodes, (T *)t->bbox,
+                        (T *)t->dbox, dec.as<Decision<T>>(), level == 0 && !t->periodic, so, cstat_r.as<Ext<T>>());
+            } else {
+                if (small_nodes)  // 16 lanes per chunk
+                    k_chunk_stats_g<T, D, STATS_G><<<(unsigned)ceil_div(nub * STATS_G, BT), BT, 0, s>>>(
+                        chunks.as<Chunk>(), d_ntot, Pc, n, cstat.as<Ext<T>>());
+                else
+                    k_chunk_stats<T, D><<<gch, BT, 0, s>>>(chunks.as<Chunk>(), d_ntot, Pc, n, cstat.as<Ext<T>>());
+                if (small_nodes && nact >= DECIDE_THREAD_MIN)  // one thread per node
+                    k_node_decide_t<T, D><<<(unsigned)ceil_div(nact, BT), BT, 0, s>>>(
+                        actc, nact, d_cb, d_nch, cstat.as<Ext<T>>(), (NodeRec<T> *)t->nodes, (T *)t->bbox, (T *)t->dbox,
+                        dec.as<Decision<T>>(), level == 0 && !t->periodic);
+                else if (nact <= DECIDE_BLOCK_MAX)  // a workgroup per node
+                    k_node_decide_b<T, D><<<(unsigned)nact, BT, 0, s>>>(
+                        actc, nact, d_cb, d_nch, cstat.as<Ext<T>>(), (NodeRec<T> *)t->nodes, (T *)t->bbox, (T *)t->dbox,
+                        dec.as<Decision<T>>(), level == 0 && !t->periodic);
+                else
+                    k_node_decide<T, D><<<(unsigned)ceil_div(nact, CPB), BT, 0, s>>>(
+                        actc, nact, d_cb, d_nch, cstat.as<Ext<T>>(), (NodeRec<T> *)t->nodes, (T *)t->bbox, (T *)t->dbox,
+                        dec.as<Decision<T>>(), level == 0 && !t->periodic);
+            }
             k_chunk_countleft<T><<<gch, BT, 0, s>>>(chunks.as<Chunk>(), d_ntot, nub, Pc, n,
                                                                dec.as<Decision<T>>(), cleft.as<int64_t>());
             if ((rc = scan_i64(cleft.as<int64_t>(), cleft_off.as<int64_t>(), nub, true, scratch.p, s))) break;
-            k_chunk_scatter<T, D, IdxT><<<gch, BT, 0, s>>>(
-                chunks.as<Chunk>(), d_ntot, d_cb, d_nch, actc, t->range, Pc, Ic, Pn, In, n, dec.as<Decision<T>>(),
-                cleft_off.as<int64_t>());
+            // levels of large nodes: the partition also records each chunk's
+            // extremes of both children (the next level's extremes pass)
+            const bool fuse_out = !small_nodes && nact <= fuse_nodes;
+            if (fuse_out)
+                k_chunk_scatter<T, D, IdxT, true><<<gch, BT, 0, s>>>(
+                    chunks.as<Chunk>(), d_ntot, d_cb, d_nch, actc, t->range, Pc, Ic, Pn, In, n,
+                    dec.as<Decision<T>>(), cleft_off.as<int64_t>(), cstat.as<Ext<T>>(), cstat_r.as<Ext<T>>());
+            else
+                k_chunk_scatter<T, D, IdxT><<<gch, BT, 0, s>>>(
+                    chunks.as<Chunk>(), d_ntot, d_cb, d_nch, actc, t->range, Pc, Ic, Pn, In, n,
+                    dec.as<Decision<T>>(), cleft_off.as<int64_t>());
             k_make_children<T, D><<<ga, 256, 0, s>>>(actc, nact, dec.as<Decision<T>>(), d_cb, d_nch,
                                                      cleft_off.as<int64_t>(), (NodeRec<T> *)t->nodes, (T *)t->bbox,
                                                      t->range, next_id, leaf, child_active.as<int64_t>());
             const int64_t m = 2 * nact;
             if ((rc = scan_i64(child_active.as<int64_t>(), child_pos.as<int64_t>(), m, true, scratch.p, s))) break;
             k_compact_active<<<(unsigned)ceil_div(m, 256), 256, 0, s>>>(
-                child_active.as<int64_t>(), child_pos.as<int64_t>(), m, next_id, actn);
+                child_active.as<int64_t>(), child_pos.as<int64_t>(), m, next_id, actn,
+                fuse_out ? slot_of.as<int64_t>() : nullptr);
+            fused_in = fuse_out;
             k_copy_leaves<T, D, IdxT><<<(unsigned)ceil_div(m * leaf, 256), 256, 0, s>>>(
                 child_active.as<int64_t>(), m, next_id, t->range, leaf, Pn, In, n, (T *)t->pts, (IdxT *)t->idx);
             CKD_HIP(hipGetLastError());
