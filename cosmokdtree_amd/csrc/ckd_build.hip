@@ -594,10 +594,18 @@ __global__ void k_copy_leaves(const int64_t *child_active, int64_t m, int32_t ne
     if (j >= r.count) return;
     const int64_t p = r.start + j;
     constexpr int PS = pstride(D);
+    T row[PS];
 #pragma unroll
-    for (int d = 0; d < D; d++) F[p * PS + d] = P[(int64_t)d * n + p];
+    for (int d = 0; d < PS; d++) row[d] = d < D ? P[(int64_t)d * n + p] : T(0);
+    if constexpr (sizeof(T) == 4 && PS == 4) {  // the row as one 16-byte store (rows are 16-byte aligned)
+        *(float4 *)(F + p * PS) = make_float4(row[0], row[1], row[2], row[3]);
+    } else if constexpr (sizeof(T) == 8 && PS == 4) {
+        *(double2 *)(F + p * PS) = make_double2(row[0], row[1]);
+        *(double2 *)(F + p * PS + 2) = make_double2(row[2], row[3]);
+    } else {
 #pragma unroll
-    for (int d = D; d < PS; d++) F[p * PS + d] = T(0);
+        for (int d = 0; d < PS; d++) F[p * PS + d] = row[d];
+    }
     FI[p] = I[p];
 }
 
