@@ -902,6 +902,25 @@ int pinned_slot(int64_t **p) {
     return CKD_OK;
 }
 
+// Waits until the copy of a count into the pinned slot *p (set to -1 before
+// the copy was queued) has landed and returns the count in *out.  A value
+// that a half-written slot could show (negative, or 2^32 - 1) keeps it
+// polling; after 2^20 polls it synchronises the stream instead, which also
+// reports an error of the queued work.
+int wait_pinned(const int64_t *p, hipStream_t s, int64_t *out) {
+    const volatile int64_t *v = p;
+    for (int i = 0; i < (1 << 20); i++) {
+        const int64_t a = *v;
+        if (a >= 0 && a != 0xFFFFFFFFLL) {
+            *out = a;
+            return CKD_OK;
+        }
+    }
+    CKD_HIP(hipStreamSynchronize(s));
+    *out = *v;
+    return CKD_OK;
+}
+
 template <typename T, int D>
 int renumber_dfs(ckd_tree *t, const std::vector<int64_t> &lvl_start, hipStream_t s) {
     const int64_t nn = t->nnodes, nsets = t->nsets;
@@ -1197,9 +1216,13 @@ int build_impl(ckd_tree *t, const T *in, hipStream_t s) {
             k_copy_leaves<T, D, IdxT><<<(unsigned)ceil_div(m * leaf, 256), 256, 0, s>>>(
                 child_active.as<int64_t>(), m, next_id, t->range, leaf, Pn, In, n, (T *)t->pts, (IdxT *)t->idx);
             CKD_HIP(hipGetLastError());
+            // the next level's node count: the copy lands in pinned memory,
+            // which the host polls (a stream synchronisation per level cost
+            // more); past a bound on the polls it synchronises after all
+            *(volatile int64_t *)h_nact = -1;
             CKD_HIP(hipMemcpyAsync(h_nact, child_pos.as<int64_t>() + m, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-            CKD_HIP(hipStreamSynchronize(s));
-            const int64_t nact_next = *h_nact;
+            int64_t nact_next = 0;
+            if ((rc = wait_pinned(h_nact, s, &nact_next))) return rc;
             nleaves += m - nact_next;
             next_id += (int32_t)m;
             lvl_start.push_back(next_id);
