@@ -1,4 +1,4 @@
-"""The large single-GPU box at full scale.
+"""The large periodic box (2^30 points, L = 500) on one GPU at full scale.
 
 N = 2^30 points in a periodic cube of side L = 500, float32 coordinates,
 64-bit stored indices (the reference's LONGINT=1), 70 % of the points in
